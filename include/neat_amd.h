@@ -1,0 +1,192 @@
+/*
+ * neat_amd.h -- C ABI of the MI355X (gfx950) differentiable forward projector.
+ *
+ * Plain pointers, sizes and a HIP stream; no torch types. Every call is asynchronous on `stream` (a
+ * hipStream_t passed as void*, NULL = the default stream) and returns a status code; on failure
+ * neat_last_error() returns a thread-local message. Device arrays are caller-allocated; functions that
+ * need scratch memory take (workspace, workspace_bytes) and publish the size through a *_workspace_bytes
+ * query. No entry point keeps state between calls, so concurrent calls on different streams are safe.
+ *
+ * Conventions shared by every entry point (they restate the reference's, see the citations):
+ *  - Octree: a complete octree of depth max_depth, numbered breadth first; node n's children are
+ *    8n+1 .. 8n+8 and child i takes the upper half on axis k iff bit k of i is set
+ *    (HyperTree.cpp:94-129). node_pos_min / node_pos_max are float [num_nodes, 3].
+ *  - node_flags: uint8 [num_nodes], bit 0 = node active, bit 1 = some strict descendant is active.
+ *    neat_tree_flags_host() derives it from the reference's node_active buffer (HyperTree.h:229-230).
+ *  - Rays: float [R, 3] origin and direction (RayList, HelperStructs.h:92-98).
+ *  - Canonical sample order: rays in input order; within a ray, segments by (tmin, node id); within a
+ *    segment, samples by index j. (The reference's order depends on atomics, HyperTree.cu:281,301.)
+ *  - Grids: float [N, C, Z, Y, X] addressed through element strides, so both the reference's contiguous
+ *    layout and the channels-last-3d layout (C stride 1) are accepted; channels-last is the fast path.
+ *  - Decoder parameters are one packed float array: for each Linear layer in order, W [out, in] row-major,
+ *    then b [out] (FCBlockImpl, ImplicitNet.h:18-71).
+ */
+#ifndef NEAT_AMD_H
+#define NEAT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ----------------------------------------------------------------------------------------------- */
+#define NEAT_OK 0
+#define NEAT_E_ARG 1         /* invalid argument: null pointer, negative size, inconsistent shape        */
+#define NEAT_E_OVERFLOW 2    /* a caller-provided buffer (workspace or output) is too small               */
+#define NEAT_E_HIP 3         /* the HIP runtime reported an error                                         */
+#define NEAT_E_UNSUPPORTED 4 /* a configuration this build has no kernel for (message names it)          */
+
+/* activation ids (Saiga::ActivationFromString names and DecodeFeatures' last activation, geometry.cpp:123-143) */
+#define NEAT_ACT_ID 0
+#define NEAT_ACT_RELU 1
+#define NEAT_ACT_SILU 2
+#define NEAT_ACT_SOFTPLUS 3
+#define NEAT_ACT_ABS 4
+
+typedef void* neat_stream_t;
+
+/* No reference counterpart (the reference aborts through glog CHECK). Thread-local message of the last
+ * failing call on this thread. */
+const char* neat_last_error(void);
+int neat_abi_version(void);
+
+/* ---- octree (host helpers, no device work) ------------------------------------------------------------------ */
+
+/* Number of nodes of the complete octree of depth max_depth (HyperTree.cpp:13-25), -1 outside [0, 9]. */
+int64_t neat_tree_num_nodes(int32_t max_depth);
+
+/* Node table of HyperTreeBaseImpl's constructor (HyperTree.cpp:11-132) into host arrays: parent [n], children
+ * [n, 8] (-1 on the deepest level), pos_min / pos_max [n, 3], scale [n], depth [n]. */
+int neat_tree_build_host(int32_t max_depth, int32_t* parent, int32_t* children, float* pos_min, float* pos_max,
+                         float* scale, int32_t* depth);
+
+/* active_node_ids (ascending ids of nodes with node_active == 1) and node_active_prefix_sum (the node's grid slot,
+ * -1 when inactive), the host loop of HyperTreeBaseImpl::UpdateActive (HyperTree.cpp:169-206). Returns the active
+ * count, -1 on bad arguments. */
+int64_t neat_tree_update_active_host(const int32_t* node_active, int32_t num_nodes, int64_t* active_ids,
+                                     int64_t* node_active_prefix_sum);
+
+/* Derives node_flags from node_active (int32 0/1 [num_nodes], HyperTree.h:229-230) for the traversal used by
+ * the ray sampler. Host memory in and out. Replaces the host loop of HyperTreeBaseImpl::UpdateActive
+ * (HyperTree.cpp:169-206) as far as the sampler needs it. */
+int neat_tree_flags_host(const int32_t* node_active, int32_t num_nodes, int32_t max_depth, uint8_t* node_flags);
+
+/* Checks on the host that a node table is the canonical breadth-first dyadic octree of depth max_depth
+ * (the table HyperTreeBaseImpl's constructor builds, HyperTree.cpp:11-132) that the sampler's traversal
+ * relies on. Returns NEAT_E_ARG, with the first mismatch in the message, otherwise NEAT_OK. */
+int neat_tree_validate_host(const float* node_pos_min, const float* node_pos_max, const int32_t* node_children,
+                            int32_t num_nodes, int32_t max_depth);
+
+/* ---- ray / leaf intersection and sample generation --------------------------------------------------------------
+ * Together these replace HyperTreeBaseImpl::CreateSamplesForRays (HyperTree.h:157, HyperTree.cu:393-438) and its
+ * kernel ComputeRaySamples (HyperTree.cu:206-391): exact allocation (count -> scan -> emit) instead of a
+ * predicted buffer, no 256-segment cap, canonical order. */
+
+size_t neat_raysample_workspace_bytes(int64_t num_rays);
+
+/* Pass 1: per-ray segment and sample counts, written as exclusive offsets [num_rays + 1] (totals at index
+ * num_rays; the caller reads them back to size pass 2). max_samples_per_ray_out (device int32, may be NULL)
+ * receives the largest per-ray sample count (SampleList::max_samples_per_ray, HyperTree.cu:302,422). */
+int neat_raysample_count(neat_stream_t stream, const float* node_pos_min, const float* node_pos_max,
+                         const uint8_t* node_flags, int32_t num_nodes, int32_t max_depth, const float* ray_origin,
+                         const float* ray_direction, int64_t num_rays, int32_t max_samples_per_node,
+                         int64_t* ray_seg_offset, int64_t* ray_sample_offset, int32_t* max_samples_per_ray_out,
+                         void* workspace, size_t workspace_bytes);
+
+/* Pass 2: the canonical segment list [num_segments]: node id, ray, sample count, tmin, tmax and the global
+ * offset of the segment's first sample. */
+int neat_raysample_segments(neat_stream_t stream, const float* node_pos_min, const float* node_pos_max,
+                            const uint8_t* node_flags, int32_t num_nodes, int32_t max_depth, const float* ray_origin,
+                            const float* ray_direction, int64_t num_rays, int32_t max_samples_per_node,
+                            const int64_t* ray_seg_offset, const int64_t* ray_sample_offset, int64_t num_segments,
+                            int32_t* seg_node, int32_t* seg_ray, int32_t* seg_count, float* seg_tmin, float* seg_tmax,
+                            int64_t* seg_sample_offset);
+
+/* Pass 3: SampleList columns (HelperStructs.h:168-189) for every sample: t, weight (midpoint cell width),
+ * global coordinate o + d*t [S,3] (may be NULL), ray index, index within the ray, node id. jitter != 0 draws the
+ * in-cell position from the counter hash of (seed, ray, segment rank, j); jitter == 0 uses the cell midpoint. */
+int neat_raysample_emit(neat_stream_t stream, const float* ray_origin, const float* ray_direction, int64_t num_rays,
+                        const int64_t* ray_seg_offset, const int64_t* ray_sample_offset, int64_t num_segments,
+                        const int32_t* seg_node, const int32_t* seg_ray, const int32_t* seg_count,
+                        const float* seg_tmin, const float* seg_tmax, const int64_t* seg_sample_offset,
+                        int64_t num_samples, int32_t jitter, uint64_t seed, float* out_t, float* out_weight,
+                        float* out_global_coordinate, int64_t* out_ray_index, int64_t* out_local_index,
+                        int64_t* out_node_id);
+
+/* ---- local coordinates and slots ---------------------------------------------------------------------------- */
+
+/* u = (g - min[node]) / (max[node] - min[node]) * 2 - 1, float [S,3] (HyperTreeBaseImpl::ComputeLocalSamples,
+ * HyperTree.h:134, HyperTree.cu:177-192). */
+int neat_local_samples(neat_stream_t stream, const float* global_coordinate, const int64_t* node_id,
+                       int64_t num_samples, const float* node_pos_min, const float* node_pos_max, int32_t num_nodes,
+                       float* out_local);
+
+/* grad_global = grad_local * 2 / (max - min): the autograd backward of the expression above. */
+int neat_local_samples_bwd(neat_stream_t stream, const float* grad_local, const int64_t* node_id,
+                           int64_t num_samples, const float* node_pos_min, const float* node_pos_max,
+                           int32_t num_nodes, float* grad_global);
+
+/* slot = node_active_prefix_sum[node_id] (GlobalNodeIdToLocalActiveId, HyperTree.h:183-187).
+ * Node ids outside [0, num_nodes) give local coordinates / gradients of 0 and slot -1 instead of reading out of
+ * bounds. */
+int neat_node_to_slot(neat_stream_t stream, const int64_t* node_id, int64_t num_samples,
+                      const int64_t* node_active_prefix_sum, int32_t num_nodes, int64_t* out_slot);
+
+/* ---- indirect trilinear grid sample ------------------------------------------------------------------------------
+ * out[i, c] = trilinear(grid[index[i], c], uv[i]) with align_corners and border clamp
+ * (IndirectGridSample3D, IndirectGridSample3D.h:17-30; kernels IndirectGridSample3D.cu:432-572 / 584-791).
+ * strides: element strides of grid dims (N, C, Z, Y, X). uv: float [S,3] contiguous; out: float [S,C]. */
+int neat_gs3d_fwd(neat_stream_t stream, const float* grid, int64_t N, int32_t C, int32_t Z, int32_t Y, int32_t X,
+                  const int64_t* strides, const int64_t* index, const float* uv, int64_t num_samples, float* out);
+
+/* grad_grid (same strides as grid) is accumulated and must be zeroed by the caller; grad_uv [S,3] is written
+ * (may be NULL). */
+int neat_gs3d_bwd(neat_stream_t stream, const float* grid, int64_t N, int32_t C, int32_t Z, int32_t Y, int32_t X,
+                  const int64_t* strides, const int64_t* index, const float* uv, int64_t num_samples,
+                  const float* grad_out, float* grad_grid, float* grad_uv);
+
+/* ---- decoder MLP ---------------------------------------------------------------------------------------------------
+ * FCBlockImpl::forward (ImplicitNet.h:61-67) followed by the last activation of DecodeFeatures
+ * (geometry.cpp:123-143): Linear(in,hidden) -> act -> [Linear(hidden,hidden) -> act] x hidden_layers ->
+ * Linear(hidden,out) -> last_act. Supported: hidden in {16,32,64}, hidden_layers in {0,1,2}, in <= 16, out <= 4. */
+int64_t neat_decoder_num_params(int32_t in, int32_t hidden, int32_t hidden_layers, int32_t out);
+
+int neat_decoder_fwd(neat_stream_t stream, const float* x, int64_t num_samples, int32_t in, int32_t hidden,
+                     int32_t hidden_layers, int32_t out, const float* params, int32_t act, int32_t last_act,
+                     float softplus_beta, float* y);
+
+size_t neat_decoder_workspace_bytes(int64_t num_samples, int32_t in, int32_t hidden, int32_t hidden_layers,
+                                    int32_t out);
+
+/* grad_params (packed, written -- not accumulated) and grad_x [S,in] (may be NULL) for upstream grad_y [S,out]. */
+int neat_decoder_bwd(neat_stream_t stream, const float* x, int64_t num_samples, int32_t in, int32_t hidden,
+                     int32_t hidden_layers, int32_t out, const float* params, int32_t act, int32_t last_act,
+                     float softplus_beta, const float* grad_y, float* grad_params, float* grad_x, void* workspace,
+                     size_t workspace_bytes);
+
+/* ---- per-ray line integral --------------------------------------------------------------------------------------
+ * out[c, ray_index[i]] += values[i, c] * weight[i]  (NeuralGeometry::IntegrateSamplesXRay, geometry.h:112-113,
+ * geometry.cpp:393-424). With ray_sample_offset [R+1] (samples of ray r are [off[r], off[r+1])) the sum is a
+ * deterministic segmented reduction; with NULL it falls back to atomics like the reference's index_add_.
+ * out [C, R] is written (not accumulated). */
+int neat_integrate_fwd(neat_stream_t stream, const float* values, const float* weight, const int64_t* ray_index,
+                       int64_t num_samples, int32_t C, int64_t num_rays, const int64_t* ray_sample_offset, float* out);
+
+/* grad_values[i, c] = grad_out[c, ray_index[i]] * weight[i]. */
+int neat_integrate_bwd(neat_stream_t stream, const float* grad_out, const float* weight, const int64_t* ray_index,
+                       int64_t num_samples, int32_t C, int64_t num_rays, float* grad_values);
+
+/* ---- loss -----------------------------------------------------------------------------------------------------------
+ * loss = mean((pred - target)^2) reduced in FP64 (reconstruct.cpp:378-381 computes it in FP32); grad_pred (may be
+ * NULL) = 2 (pred - target) / n * grad_scale. out_loss: one device double. */
+size_t neat_mse_workspace_bytes(int64_t n);
+int neat_mse_fp64(neat_stream_t stream, const float* pred, const float* target, int64_t n, float grad_scale,
+                  double* out_loss, float* grad_pred, void* workspace, size_t workspace_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEAT_AMD_H */

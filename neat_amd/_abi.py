@@ -1,0 +1,104 @@
+"""ctypes binding of libneat_amd.so (the C ABI declared in include/neat_amd.h).
+
+torch is imported first so that the HIP runtime torch ships is the one the library binds to. Loading fails loudly
+when the library has not been built: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads torch's HIP runtime before libneat_amd)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libneat_amd.so")
+
+NEAT_OK, NEAT_E_ARG, NEAT_E_OVERFLOW, NEAT_E_HIP, NEAT_E_UNSUPPORTED = 0, 1, 2, 3, 4
+ACT = {"id": 0, "relu": 1, "silu": 2, "softplus": 3, "abs": 4}
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_U64 = ctypes.c_uint64
+_F = ctypes.c_float
+_SZ = ctypes.c_size_t
+
+SIGNATURES = {
+    "neat_last_error": (ctypes.c_char_p, []),
+    "neat_abi_version": (ctypes.c_int, []),
+    "neat_tree_num_nodes": (_I64, [_I32]),
+    "neat_tree_build_host": (ctypes.c_int, [_I32, _P, _P, _P, _P, _P, _P]),
+    "neat_tree_update_active_host": (_I64, [_P, _I32, _P, _P]),
+    "neat_tree_flags_host": (ctypes.c_int, [_P, _I32, _I32, _P]),
+    "neat_tree_validate_host": (ctypes.c_int, [_P, _P, _P, _I32, _I32]),
+    "neat_raysample_workspace_bytes": (_SZ, [_I64]),
+    "neat_raysample_count": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P, _P, _I64, _I32, _P, _P, _P, _P, _SZ]),
+    "neat_raysample_segments": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P, _P, _I64, _I32, _P, _P, _I64,
+                                               _P, _P, _P, _P, _P, _P]),
+    "neat_raysample_emit": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I32, _U64,
+                                           _P, _P, _P, _P, _P, _P]),
+    "neat_local_samples": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _I32, _P]),
+    "neat_local_samples_bwd": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _I32, _P]),
+    "neat_node_to_slot": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P]),
+    "neat_gs3d_fwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _P]),
+    "neat_gs3d_bwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _P, _P, _P]),
+    "neat_decoder_num_params": (_I64, [_I32, _I32, _I32, _I32]),
+    "neat_decoder_fwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _I32, _I32, _P, _I32, _I32, _F, _P]),
+    "neat_decoder_workspace_bytes": (_SZ, [_I64, _I32, _I32, _I32, _I32]),
+    "neat_decoder_bwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _I32, _I32, _P, _I32, _I32, _F, _P, _P, _P,
+                                        _P, _SZ]),
+    "neat_integrate_fwd": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I32, _I64, _P, _P]),
+    "neat_integrate_bwd": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I32, _I64, _P]),
+    "neat_mse_workspace_bytes": (_SZ, [_I64]),
+    "neat_mse_fp64": (ctypes.c_int, [_P, _P, _P, _I64, _F, _P, _P, _P, _SZ]),
+}
+
+_LIB = None
+
+
+class NeatError(RuntimeError):
+    def __init__(self, code, fn, msg):
+        super().__init__(f"{fn} failed with status {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m neat_amd.build` (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def last_error():
+    return lib().neat_last_error().decode()
+
+
+def check(code, fn):
+    if code != NEAT_OK:
+        raise NeatError(code, fn, last_error())
+
+
+def ptr(t):
+    """Device (or host) address of a tensor, None for None."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def call(name, *args):
+    """Calls an entry point and raises NeatError on a non-zero status."""
+    f = getattr(lib(), name)
+    r = f(*args)
+    if f.restype is ctypes.c_int:
+        check(r, name)
+    return r

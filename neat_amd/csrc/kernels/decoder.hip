@@ -1,0 +1,195 @@
+// Decoder MLP (FCBlockImpl, ImplicitNet.h:18-71) fused with DecodeFeatures' last activation (geometry.cpp:123-143),
+// forward and backward in FP32, one wave per tile of 64 samples (see mlp_tile.hpp).
+//
+// Backward: persistent workgroups; each recomputes its tiles' forward, back-propagates, and accumulates its share
+// of the parameter gradient into its own slice of a partials buffer (every entry owned by one thread); a second
+// kernel sums the slices in workgroup order, so the result is deterministic.
+#include "mlp_tile.hpp"
+
+namespace neat
+{
+
+__global__ void __launch_bounds__(256) k_transpose_params(const float* params, MLP m, float* wt)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m.nparams()) return;
+    int l = 0;
+    while (l + 1 < m.nlin() && p >= m.wofs(l + 1)) ++l;
+    int rel = p - m.wofs(l);
+    int din = m.din(l), dout = m.dout(l);
+    if (rel < din * dout)
+    {
+        int o = rel / din, k = rel % din;       // params: W[o][k]
+        wt[m.wofs(l) + k * dout + o] = params[p];  // wt: Wt[k][o]
+    }
+    else
+    {
+        wt[p] = params[p];  // bias
+    }
+}
+
+__device__ __forceinline__ void load_x_col(const float* x, int64_t i0, int ns, int in, float* X)
+{
+    const int s = threadIdx.x & (kTile - 1);
+#pragma unroll
+    for (int k = 0; k < kMaxIn; ++k) X[k * kS + s] = (s < ns && k < in) ? x[(i0 + s) * in + k] : 0.f;
+}
+
+template <int H>
+__global__ void __launch_bounds__(kTile) k_dec_fwd(const float* x, int64_t S, MLP m, const float* __restrict__ wt,
+                                                   int act, int last_act, float beta, float* y)
+{
+    __shared__ float lds[kTileFloats];
+    TileBufs t       = tile_bufs(lds);
+    const int64_t i0 = (int64_t)blockIdx.x * kTile;
+    const int ns     = (int)min<int64_t>(kTile, S - i0);
+    const int s      = threadIdx.x;
+    load_x_col(x, i0, ns, m.in, t.X);
+    __syncthreads();
+    tile_forward<H>(m, wt, WLayout{false}, act, beta, t);
+    if (s < ns)
+    {
+        for (int o = 0; o < m.out; ++o) y[(i0 + s) * m.out + o] = act_f(last_act, t.D[o * kS + s], beta);
+    }
+}
+
+template <int H>
+__global__ void __launch_bounds__(kTile) k_dec_bwd(const float* x, int64_t S, MLP m, const float* __restrict__ wt,
+                                                   int act, int last_act, float beta, const float* gy, float* gx,
+                                                   float* partials)
+{
+    __shared__ float lds[kTileFloats];
+    TileBufs t          = tile_bufs(lds);
+    float* gacc         = partials + (int64_t)blockIdx.x * m.nparams();
+    const int64_t tiles = (S + kTile - 1) / kTile;
+    const int s         = threadIdx.x;
+    for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    {
+        const int64_t i0 = tile * kTile;
+        const int ns     = (int)min<int64_t>(kTile, S - i0);
+        __syncthreads();
+        load_x_col(x, i0, ns, m.in, t.X);
+        __syncthreads();
+        tile_forward<H>(m, wt, WLayout{true}, act, beta, t);
+        for (int o = 0; o < m.out; ++o)
+        {
+            float* p = t.D + o * kS + s;
+            *p       = s < ns ? gy[(i0 + s) * m.out + o] * act_d(last_act, *p, beta) : 0.f;
+        }
+        tile_backward<H>(m, wt, WLayout{true}, act, beta, t, ns, gacc);
+        if (gx && s < ns)
+        {
+            for (int k = 0; k < m.in; ++k) gx[(i0 + s) * m.in + k] = t.X[k * kS + s];
+        }
+    }
+}
+
+// Fixed-order sum of the workgroup partials.
+__global__ void __launch_bounds__(256) k_dec_reduce(const float* partials, int num_wg, int P, float* out)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float acc = 0.f;
+    for (int w = 0; w < num_wg; ++w) acc += partials[(int64_t)w * P + p];
+    out[p] = acc;
+}
+
+constexpr int kMaxBwdWG = 1024;
+
+int bwd_workgroups(int64_t S)
+{
+    int64_t tiles = (S + kTile - 1) / kTile;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, kMaxBwdWG));
+}
+
+int check_mlp(int in, int hidden, int layers, int out, int act, int last_act)
+{
+    NEAT_REQUIRE(in >= 1 && in <= kMaxIn, "decoder in_features %d outside [1, %d]", in, kMaxIn);
+    NEAT_REQUIRE(out >= 1 && out <= kMaxOut, "decoder out_features %d outside [1, %d]", out, kMaxOut);
+    NEAT_REQUIRE(act >= 0 && act <= 4 && last_act >= 0 && last_act <= 4, "unknown activation id");
+    if (!((hidden == 16 || hidden == 32 || hidden == 64) && (layers == 0 || layers == 1)))
+        return fail(NEAT_E_UNSUPPORTED,
+                    "decoder hidden=%d hidden_layers=%d: this build supports hidden {16,32,64} and hidden_layers {0,1}",
+                    hidden, layers);
+    return NEAT_OK;
+}
+
+}  // namespace neat
+
+using namespace neat;
+
+extern "C"
+{
+    int64_t neat_decoder_num_params(int32_t in, int32_t hidden, int32_t layers, int32_t out)
+    {
+        MLP m{in, hidden, layers, out};
+        return m.nparams();
+    }
+
+    size_t neat_decoder_workspace_bytes(int64_t S, int32_t in, int32_t hidden, int32_t layers, int32_t out)
+    {
+        MLP m{in, hidden, layers, out};
+        size_t P = (size_t)m.nparams();
+        return (P + (size_t)bwd_workgroups(S) * P) * sizeof(float);
+    }
+
+    int neat_decoder_fwd(neat_stream_t stream, const float* x, int64_t S, int32_t in, int32_t hidden, int32_t layers,
+                         int32_t out, const float* params, int32_t act, int32_t last_act, float beta, float* y)
+    {
+        int st = check_mlp(in, hidden, layers, out, act, last_act);
+        if (st) return st;
+        NEAT_REQUIRE(S >= 0, "num_samples < 0");
+        if (S == 0) return NEAT_OK;
+        NEAT_REQUIRE(x && params && y, "neat_decoder_fwd: null pointer");
+        MLP m{in, hidden, layers, out};
+        hipStream_t s = as_stream(stream);
+        const float* wt = params;  // packed layout read in place
+        unsigned grid   = grid_for(S, kTile);
+        if (hidden == 16)
+            k_dec_fwd<16><<<grid, kTile, 0, s>>>(x, S, m, wt, act, last_act, beta, y);
+        else if (hidden == 32)
+            k_dec_fwd<32><<<grid, kTile, 0, s>>>(x, S, m, wt, act, last_act, beta, y);
+        else
+            k_dec_fwd<64><<<grid, kTile, 0, s>>>(x, S, m, wt, act, last_act, beta, y);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
+
+    int neat_decoder_bwd(neat_stream_t stream, const float* x, int64_t S, int32_t in, int32_t hidden, int32_t layers,
+                         int32_t out, const float* params, int32_t act, int32_t last_act, float beta, const float* gy,
+                         float* gparams, float* gx, void* workspace, size_t workspace_bytes)
+    {
+        int st = check_mlp(in, hidden, layers, out, act, last_act);
+        if (st) return st;
+        NEAT_REQUIRE(S >= 0, "num_samples < 0");
+        NEAT_REQUIRE(params && gparams, "neat_decoder_bwd: null pointer");
+        MLP m{in, hidden, layers, out};
+        const int P   = m.nparams();
+        hipStream_t s = as_stream(stream);
+        if (S == 0)
+        {
+            NEAT_HIP(hipMemsetAsync(gparams, 0, (size_t)P * sizeof(float), s));
+            return NEAT_OK;
+        }
+        NEAT_REQUIRE(x && gy, "neat_decoder_bwd: null pointer");
+        size_t need = neat_decoder_workspace_bytes(S, in, hidden, layers, out);
+        if (!workspace || workspace_bytes < need)
+            return fail(NEAT_E_OVERFLOW, "decoder workspace too small: %zu < %zu", workspace_bytes, need);
+        float* wt       = static_cast<float*>(workspace);
+        float* partials = wt + P;
+        const int nwg   = bwd_workgroups(S);
+        k_transpose_params<<<grid_for(P, 256), 256, 0, s>>>(params, m, wt);
+        NEAT_LAUNCHED();
+        NEAT_HIP(hipMemsetAsync(partials, 0, (size_t)nwg * P * sizeof(float), s));
+        if (hidden == 16)
+            k_dec_bwd<16><<<nwg, kTile, 0, s>>>(x, S, m, wt, act, last_act, beta, gy, gx, partials);
+        else if (hidden == 32)
+            k_dec_bwd<32><<<nwg, kTile, 0, s>>>(x, S, m, wt, act, last_act, beta, gy, gx, partials);
+        else
+            k_dec_bwd<64><<<nwg, kTile, 0, s>>>(x, S, m, wt, act, last_act, beta, gy, gx, partials);
+        NEAT_LAUNCHED();
+        k_dec_reduce<<<grid_for(P, 256), 256, 0, s>>>(partials, nwg, P, gparams);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
+}

@@ -1,0 +1,282 @@
+// Ray / active-leaf intersection and per-ray sample generation (replaces ComputeRaySamples, HyperTree.cu:206-391,
+// and CreateSamplesForRays, HyperTree.cu:393-438).
+//
+// Instead of testing every active leaf per ray (R x A slab tests) each thread walks the octree top-down and only
+// visits children of nodes the ray hits. The pruning is exact: a child box lies inside its parent box, and the
+// correctly rounded subtraction and division in the slab test are monotonic, so the child's per-axis interval is a
+// sub-interval of the parent's; a parent miss therefore implies a miss for every descendant, and every active node
+// the brute-force loop would report is visited with the very same slab test. The walk is stackless: in the
+// breadth-first numbering node n's children are 8n+1..8n+8.
+//
+// Output order is canonical (segments of a ray sorted by (tmin, node id)); the whole file is compiled without
+// floating-point contraction so counts and t values are bit-identical to the oracle.
+#include "common.hpp"
+
+namespace neat
+{
+
+struct TreeArgs
+{
+    const float* pmin;
+    const float* pmax;
+    const uint8_t* flags;  // bit 0: active, bit 1: has an active strict descendant
+    int32_t first_leaf;    // first node of the deepest level: nodes below it have children 8n+1..8n+8
+};
+
+// Sample count of a segment (HyperTree.cu:287-291): iCeil(|tmax - tmin| / |max - min|_2 * max_spn), the norm added
+// as x + (y + z) like Eigen's fixed-size reduction.
+__device__ __forceinline__ int32_t segment_count(const float mn[3], const float mx[3], float tmin, float tmax,
+                                                 int32_t max_spn)
+{
+    float ex   = mx[0] - mn[0];
+    float ey   = mx[1] - mn[1];
+    float ez   = mx[2] - mn[2];
+    float diag = sqrtf(ex * ex + (ey * ey + ez * ez));
+    float dis  = tmax - tmin;
+    float rel  = dis / diag;
+    return (int32_t)ceilf(rel * (float)max_spn);
+}
+
+template <typename Visit>
+__device__ __forceinline__ void traverse(const TreeArgs& T, const float o[3], const float d[3], Visit&& visit)
+{
+    int32_t n = 0;
+    while (true)
+    {
+        uint8_t f    = T.flags[n];
+        bool descend = false;
+        if (f)
+        {
+            float mn[3] = {T.pmin[3 * n], T.pmin[3 * n + 1], T.pmin[3 * n + 2]};
+            float mx[3] = {T.pmax[3 * n], T.pmax[3 * n + 1], T.pmax[3 * n + 2]};
+            Hit h       = slab_test(mn, mx, o, d);
+            if (h.hit)
+            {
+                // sliver filter (HyperTree.cu:274): float difference compared in double
+                if ((f & 1) && !((double)(h.tmax - h.tmin) < 1e-5)) visit(n, h.tmin, h.tmax, mn, mx);
+                descend = (f & 2) != 0 && n < T.first_leaf;  // never leave the node table
+            }
+        }
+        if (descend)
+        {
+            n = 8 * n + 1;
+            continue;
+        }
+        while (n != 0 && ((n - 1) & 7) == 7) n = (n - 1) >> 3;
+        if (n == 0) break;
+        ++n;
+    }
+}
+
+__device__ __forceinline__ void load_ray(const float* ro, const float* rd, int64_t r, float o[3], float d[3])
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        o[k] = ro[3 * r + k];
+        d[k] = rd[3 * r + k];
+    }
+}
+
+__global__ void k_zero_i32(int32_t* p)
+{
+    if (p) *p = 0;
+}
+
+__global__ void __launch_bounds__(128) k_ray_count(TreeArgs T, const float* ro, const float* rd, int64_t R,
+                                                   int32_t max_spn, int64_t* nseg, int64_t* nsamp, int32_t* max_per_ray)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    float o[3], d[3];
+    load_ray(ro, rd, r, o, d);
+    int64_t ns = 0, nk = 0;
+    traverse(T, o, d,
+             [&](int32_t, float tmin, float tmax, const float* mn, const float* mx)
+             {
+                 nk += 1;
+                 ns += segment_count(mn, mx, tmin, tmax, max_spn);
+             });
+    nseg[r]  = nk;
+    nsamp[r] = ns;
+    if (max_per_ray) atomicMax(max_per_ray, (int32_t)ns);
+}
+
+__device__ __forceinline__ bool seg_less(float ta, int32_t na, float tb, int32_t nb)
+{
+    return ta < tb || (ta == tb && na < nb);
+}
+
+__global__ void __launch_bounds__(128) k_ray_segments(TreeArgs T, const float* ro, const float* rd, int64_t R,
+                                                      int32_t max_spn, const int64_t* ray_seg_offset,
+                                                      const int64_t* ray_sample_offset, int32_t* seg_node,
+                                                      int32_t* seg_ray, int32_t* seg_count, float* seg_tmin,
+                                                      float* seg_tmax, int64_t* seg_sample_offset)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    float o[3], d[3];
+    load_ray(ro, rd, r, o, d);
+    const int64_t base = ray_seg_offset[r];
+    const int64_t end  = ray_seg_offset[r + 1];
+    int64_t k          = base;
+    // insertion into the sorted prefix [base, k) while walking (segments per ray are few)
+    traverse(T, o, d,
+             [&](int32_t node, float tmin, float tmax, const float* mn, const float* mx)
+             {
+                 if (k >= end) return;  // cannot happen when the offsets come from k_ray_count
+                 int32_t cnt = segment_count(mn, mx, tmin, tmax, max_spn);
+                 int64_t p   = k;
+                 while (p > base && seg_less(tmin, node, seg_tmin[p - 1], seg_node[p - 1]))
+                 {
+                     seg_tmin[p]  = seg_tmin[p - 1];
+                     seg_tmax[p]  = seg_tmax[p - 1];
+                     seg_node[p]  = seg_node[p - 1];
+                     seg_count[p] = seg_count[p - 1];
+                     --p;
+                 }
+                 seg_tmin[p]  = tmin;
+                 seg_tmax[p]  = tmax;
+                 seg_node[p]  = node;
+                 seg_count[p] = cnt;
+                 ++k;
+             });
+    int64_t so = ray_sample_offset[r];
+    for (int64_t q = base; q < end; ++q)
+    {
+        seg_ray[q]           = (int32_t)r;
+        seg_sample_offset[q] = so;
+        so += seg_count[q];
+    }
+}
+
+// One thread per sample; the owning segment is the last one whose first sample is <= i.
+__global__ void __launch_bounds__(256) k_ray_emit(const float* ro, const float* rd, const int64_t* ray_seg_offset,
+                                                  const int64_t* ray_sample_offset, int64_t K, const int32_t* seg_node,
+                                                  const int32_t* seg_ray, const int32_t* seg_count,
+                                                  const float* seg_tmin, const float* seg_tmax,
+                                                  const int64_t* seg_sample_offset, int64_t S, int32_t jitter,
+                                                  uint64_t seed, float* out_t, float* out_w, float* out_coord,
+                                                  int64_t* out_ray, int64_t* out_local, int64_t* out_node)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    int64_t lo = 0, hi = K - 1;
+    while (lo < hi)
+    {
+        int64_t mid = (lo + hi + 1) >> 1;
+        if (seg_sample_offset[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int64_t k    = lo;
+    const int64_t ray  = seg_ray[k];
+    const int32_t n    = seg_count[k];
+    const int32_t j    = (int32_t)(i - seg_sample_offset[k]);
+    const int32_t rank = (int32_t)(k - ray_seg_offset[ray]);
+    const float tmin = seg_tmin[k], tmax = seg_tmax[k];
+    const float step = (tmax - tmin) / (float)n;
+    const float t    = sample_t(tmin, step, j, seg_a(jitter, seed, ray, rank, j));
+    out_t[i]         = t;
+    out_w[i]         = sample_w(tmin, tmax, step, n, j, t, jitter, seed, ray, rank);
+    if (out_coord)
+    {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out_coord[3 * i + c] = ro[3 * ray + c] + rd[3 * ray + c] * t;
+    }
+    out_ray[i]   = ray;
+    out_local[i] = i - ray_sample_offset[ray];
+    out_node[i]  = seg_node[k];
+}
+
+}  // namespace neat
+
+using namespace neat;
+
+static int32_t first_leaf(int32_t num_nodes) { return (num_nodes - 1) / 8; }
+
+static int check_tree(const float* pmin, const float* pmax, const uint8_t* flags, int32_t num_nodes, int32_t max_depth)
+{
+    NEAT_REQUIRE(pmin && pmax && flags, "octree pointers must not be null");
+    NEAT_REQUIRE(max_depth >= 0 && max_depth <= 9, "max_depth %d outside [0, 9]", max_depth);
+    int64_t expect = 0, p = 1;
+    for (int d = 0; d <= max_depth; ++d, p *= 8) expect += p;
+    NEAT_REQUIRE(num_nodes == expect, "num_nodes %d is not a complete octree of depth %d (%lld)", num_nodes, max_depth,
+                 (long long)expect);
+    return NEAT_OK;
+}
+
+extern "C"
+{
+    size_t neat_raysample_workspace_bytes(int64_t num_rays) { return scan_workspace_bytes(num_rays); }
+
+    int neat_raysample_count(neat_stream_t stream, const float* pmin, const float* pmax, const uint8_t* flags,
+                             int32_t num_nodes, int32_t max_depth, const float* ro, const float* rd, int64_t R,
+                             int32_t max_spn, int64_t* ray_seg_offset, int64_t* ray_sample_offset,
+                             int32_t* max_per_ray, void* workspace, size_t workspace_bytes)
+    {
+        int st = check_tree(pmin, pmax, flags, num_nodes, max_depth);
+        if (st) return st;
+        NEAT_REQUIRE(R >= 0, "num_rays %lld < 0", (long long)R);
+        NEAT_REQUIRE(max_spn > 0, "max_samples_per_node %d must be > 0", max_spn);
+        NEAT_REQUIRE(ray_seg_offset && ray_sample_offset, "offset arrays must not be null");
+        NEAT_REQUIRE(R == 0 || (ro && rd), "ray arrays must not be null");
+        hipStream_t s = as_stream(stream);
+        k_zero_i32<<<1, 1, 0, s>>>(max_per_ray);
+        NEAT_LAUNCHED();
+        if (R > 0)
+        {
+            TreeArgs T{pmin, pmax, flags, first_leaf(num_nodes)};
+            k_ray_count<<<grid_for(R, 128), 128, 0, s>>>(T, ro, rd, R, max_spn, ray_seg_offset, ray_sample_offset,
+                                                          max_per_ray);
+            NEAT_LAUNCHED();
+        }
+        st = exclusive_scan_i64(s, ray_seg_offset, R, workspace, workspace_bytes);
+        if (st) return st;
+        return exclusive_scan_i64(s, ray_sample_offset, R, workspace, workspace_bytes);
+    }
+
+    int neat_raysample_segments(neat_stream_t stream, const float* pmin, const float* pmax, const uint8_t* flags,
+                                int32_t num_nodes, int32_t max_depth, const float* ro, const float* rd, int64_t R,
+                                int32_t max_spn, const int64_t* ray_seg_offset, const int64_t* ray_sample_offset,
+                                int64_t K, int32_t* seg_node, int32_t* seg_ray, int32_t* seg_count, float* seg_tmin,
+                                float* seg_tmax, int64_t* seg_sample_offset)
+    {
+        int st = check_tree(pmin, pmax, flags, num_nodes, max_depth);
+        if (st) return st;
+        NEAT_REQUIRE(R >= 0 && K >= 0, "negative size");
+        NEAT_REQUIRE(max_spn > 0, "max_samples_per_node %d must be > 0", max_spn);
+        if (R == 0 || K == 0) return NEAT_OK;
+        NEAT_REQUIRE(ro && rd && ray_seg_offset && ray_sample_offset && seg_node && seg_ray && seg_count && seg_tmin &&
+                         seg_tmax && seg_sample_offset,
+                     "neat_raysample_segments: null pointer");
+        TreeArgs T{pmin, pmax, flags, first_leaf(num_nodes)};
+        k_ray_segments<<<grid_for(R, 128), 128, 0, as_stream(stream)>>>(
+            T, ro, rd, R, max_spn, ray_seg_offset, ray_sample_offset, seg_node, seg_ray, seg_count, seg_tmin, seg_tmax,
+            seg_sample_offset);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
+
+    int neat_raysample_emit(neat_stream_t stream, const float* ro, const float* rd, int64_t R,
+                            const int64_t* ray_seg_offset, const int64_t* ray_sample_offset, int64_t K,
+                            const int32_t* seg_node, const int32_t* seg_ray, const int32_t* seg_count,
+                            const float* seg_tmin, const float* seg_tmax, const int64_t* seg_sample_offset, int64_t S,
+                            int32_t jitter, uint64_t seed, float* out_t, float* out_w, float* out_coord,
+                            int64_t* out_ray, int64_t* out_local, int64_t* out_node)
+    {
+        NEAT_REQUIRE(R >= 0 && K >= 0 && S >= 0, "negative size");
+        if (S == 0) return NEAT_OK;
+        NEAT_REQUIRE(K > 0, "num_samples %lld > 0 with no segments", (long long)S);
+        NEAT_REQUIRE(ro && rd && ray_seg_offset && ray_sample_offset && seg_node && seg_ray && seg_count && seg_tmin &&
+                         seg_tmax && seg_sample_offset && out_t && out_w && out_ray && out_local && out_node,
+                     "neat_raysample_emit: null pointer");
+        k_ray_emit<<<grid_for(S, 256), 256, 0, as_stream(stream)>>>(ro, rd, ray_seg_offset, ray_sample_offset, K,
+                                                                     seg_node, seg_ray, seg_count, seg_tmin, seg_tmax,
+                                                                     seg_sample_offset, S, jitter, seed, out_t, out_w,
+                                                                     out_coord, out_ray, out_local, out_node);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
+}

@@ -1,0 +1,157 @@
+"""Thin torch-tensor wrappers over the C ABI (allocation and stream plumbing only; all compute is in the HIP library).
+
+These are what the parity tests drive directly; the libtorch module layer (_neat_host) calls the same entry points.
+"""
+import numpy as np
+import torch
+
+from . import _abi
+from ._abi import call, ptr, stream_ptr
+
+_I64 = torch.int64
+_I32 = torch.int32
+_F32 = torch.float32
+
+
+def _dev(t):
+    assert t.is_cuda, "neat_amd runs on the GPU only"
+    return t
+
+
+def tree_flags(node_active, max_depth):
+    """uint8 flags per node (bit 0 active, bit 1 active descendant), computed on the host."""
+    na = np.ascontiguousarray(np.asarray(node_active.cpu() if torch.is_tensor(node_active) else node_active),
+                              dtype=np.int32)
+    out = np.empty(na.shape[0], np.uint8)
+    _abi.check(_abi.lib().neat_tree_flags_host(na.ctypes.data, na.shape[0], max_depth, out.ctypes.data),
+               "neat_tree_flags_host")
+    return torch.from_numpy(out)
+
+
+def tree_validate(pos_min, pos_max, children, max_depth):
+    pmin = np.ascontiguousarray(pos_min, np.float32)
+    pmax = np.ascontiguousarray(pos_max, np.float32)
+    ch = np.ascontiguousarray(children, np.int32)
+    return _abi.lib().neat_tree_validate_host(pmin.ctypes.data, pmax.ctypes.data, ch.ctypes.data, pmin.shape[0],
+                                              max_depth)
+
+
+def raysample(pos_min, pos_max, flags, max_depth, origin, direction, max_spn, jitter=False, seed=0, emit=True,
+              stream=None):
+    """Canonical segment list (+ SampleList columns when emit) for rays [R,3] on the GPU."""
+    dev = origin.device
+    s = stream_ptr(stream)
+    R = origin.shape[0]
+    num_nodes = pos_min.shape[0]
+    o = _dev(origin.contiguous())
+    d = _dev(direction.contiguous())
+    seg_off = torch.empty(R + 1, dtype=_I64, device=dev)
+    samp_off = torch.empty(R + 1, dtype=_I64, device=dev)
+    maxr = torch.empty(1, dtype=_I32, device=dev)
+    ws_bytes = _abi.lib().neat_raysample_workspace_bytes(R)
+    ws = torch.empty(max(ws_bytes, 8), dtype=torch.uint8, device=dev)
+    call("neat_raysample_count", s, ptr(pos_min), ptr(pos_max), ptr(flags), num_nodes, max_depth, ptr(o), ptr(d), R,
+         max_spn, ptr(seg_off), ptr(samp_off), ptr(maxr), ptr(ws), ws_bytes)
+    K = int(seg_off[R].item())
+    S = int(samp_off[R].item())
+    r = dict(ray_seg_offset=seg_off, ray_sample_offset=samp_off, max_samples_per_ray=int(maxr.item()),
+             seg_node=torch.empty(K, dtype=_I32, device=dev), seg_ray=torch.empty(K, dtype=_I32, device=dev),
+             seg_count=torch.empty(K, dtype=_I32, device=dev), seg_tmin=torch.empty(K, dtype=_F32, device=dev),
+             seg_tmax=torch.empty(K, dtype=_F32, device=dev), seg_sample_offset=torch.empty(K, dtype=_I64, device=dev))
+    call("neat_raysample_segments", s, ptr(pos_min), ptr(pos_max), ptr(flags), num_nodes, max_depth, ptr(o), ptr(d), R,
+         max_spn, ptr(seg_off), ptr(samp_off), K, ptr(r["seg_node"]), ptr(r["seg_ray"]), ptr(r["seg_count"]),
+         ptr(r["seg_tmin"]), ptr(r["seg_tmax"]), ptr(r["seg_sample_offset"]))
+    if emit:
+        r.update(t=torch.empty(S, dtype=_F32, device=dev), weight=torch.empty(S, dtype=_F32, device=dev),
+                 coord=torch.empty(S, 3, dtype=_F32, device=dev), ray_index=torch.empty(S, dtype=_I64, device=dev),
+                 local_index=torch.empty(S, dtype=_I64, device=dev), node_id=torch.empty(S, dtype=_I64, device=dev))
+        call("neat_raysample_emit", s, ptr(o), ptr(d), R, ptr(seg_off), ptr(samp_off), K, ptr(r["seg_node"]),
+             ptr(r["seg_ray"]), ptr(r["seg_count"]), ptr(r["seg_tmin"]), ptr(r["seg_tmax"]),
+             ptr(r["seg_sample_offset"]), S, int(bool(jitter)), seed, ptr(r["t"]), ptr(r["weight"]), ptr(r["coord"]),
+             ptr(r["ray_index"]), ptr(r["local_index"]), ptr(r["node_id"]))
+    return r
+
+
+def local_samples(coord, node_id, pos_min, pos_max, stream=None):
+    out = torch.empty_like(coord)
+    call("neat_local_samples", stream_ptr(stream), ptr(coord.contiguous()), ptr(node_id.contiguous()),
+         node_id.shape[0], ptr(pos_min), ptr(pos_max), pos_min.shape[0], ptr(out))
+    return out
+
+
+def node_to_slot(node_id, prefix, stream=None):
+    out = torch.empty_like(node_id)
+    call("neat_node_to_slot", stream_ptr(stream), ptr(node_id.contiguous()), node_id.shape[0], ptr(prefix),
+         prefix.shape[0], ptr(out))
+    return out
+
+
+def _strides(grid):
+    return np.ascontiguousarray(grid.stride(), dtype=np.int64)
+
+
+def gs3d_fwd(grid, index, uv, stream=None):
+    N, C, Z, Y, X = grid.shape
+    st = _strides(grid)
+    out = torch.empty(index.shape[0], C, dtype=_F32, device=grid.device)
+    call("neat_gs3d_fwd", stream_ptr(stream), ptr(grid), N, C, Z, Y, X, st.ctypes.data, ptr(index.contiguous()),
+         ptr(uv.contiguous()), index.shape[0], ptr(out))
+    return out
+
+
+def gs3d_bwd(grid, index, uv, grad_out, want_uv=True, stream=None):
+    N, C, Z, Y, X = grid.shape
+    st = _strides(grid)
+    gg = torch.zeros_like(grid)  # preserves the grid's memory format, so the strides match
+    assert gg.stride() == grid.stride()
+    guv = torch.empty(index.shape[0], 3, dtype=_F32, device=grid.device) if want_uv else None
+    call("neat_gs3d_bwd", stream_ptr(stream), ptr(grid), N, C, Z, Y, X, st.ctypes.data, ptr(index.contiguous()),
+         ptr(uv.contiguous()), index.shape[0], ptr(grad_out.contiguous()), ptr(gg), ptr(guv))
+    return gg, guv
+
+
+def decoder_fwd(x, params, d_in, hidden, layers, d_out, act="relu", last_act="softplus", beta=2.0, stream=None):
+    y = torch.empty(x.shape[0], d_out, dtype=_F32, device=x.device)
+    call("neat_decoder_fwd", stream_ptr(stream), ptr(x.contiguous()), x.shape[0], d_in, hidden, layers, d_out,
+         ptr(params.contiguous()), _abi.ACT[act], _abi.ACT[last_act], beta, ptr(y))
+    return y
+
+
+def decoder_bwd(x, params, grad_y, d_in, hidden, layers, d_out, act="relu", last_act="softplus", beta=2.0,
+                stream=None):
+    S = x.shape[0]
+    gp = torch.empty_like(params)
+    gx = torch.empty_like(x)
+    wsb = _abi.lib().neat_decoder_workspace_bytes(S, d_in, hidden, layers, d_out)
+    ws = torch.empty(max(wsb, 8), dtype=torch.uint8, device=x.device)
+    call("neat_decoder_bwd", stream_ptr(stream), ptr(x.contiguous()), S, d_in, hidden, layers, d_out,
+         ptr(params.contiguous()), _abi.ACT[act], _abi.ACT[last_act], beta, ptr(grad_y.contiguous()), ptr(gp), ptr(gx),
+         ptr(ws), wsb)
+    return gp, gx
+
+
+def integrate_fwd(values, weight, ray_index, num_rays, ray_sample_offset=None, stream=None):
+    C = values.shape[1]
+    out = torch.empty(C, num_rays, dtype=_F32, device=values.device)
+    call("neat_integrate_fwd", stream_ptr(stream), ptr(values.contiguous()), ptr(weight.contiguous()),
+         ptr(ray_index.contiguous()), values.shape[0], C, num_rays, ptr(ray_sample_offset), ptr(out))
+    return out
+
+
+def integrate_bwd(grad_out, weight, ray_index, stream=None):
+    C, R = grad_out.shape
+    gv = torch.empty(weight.shape[0], C, dtype=_F32, device=grad_out.device)
+    call("neat_integrate_bwd", stream_ptr(stream), ptr(grad_out.contiguous()), ptr(weight.contiguous()),
+         ptr(ray_index.contiguous()), weight.shape[0], C, R, ptr(gv))
+    return gv
+
+
+def mse_fp64(pred, target, grad_scale=1.0, want_grad=True, stream=None):
+    n = pred.numel()
+    loss = torch.empty(1, dtype=torch.float64, device=pred.device)
+    grad = torch.empty_like(pred) if want_grad else None
+    wsb = _abi.lib().neat_mse_workspace_bytes(n)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=pred.device)
+    call("neat_mse_fp64", stream_ptr(stream), ptr(pred.contiguous()), ptr(target.contiguous()), n, grad_scale,
+         ptr(loss), ptr(grad), ptr(ws), wsb)
+    return loss, grad

@@ -1,0 +1,475 @@
+"""Pins the CPU oracle against what the mathematics fixes and against library routines (no reference golden vectors
+exist for this path: the reference ships no tests or fixtures).
+
+Each check is chosen so that a plausible mistake in the oracle -- a swapped min/max, a wrong axis or sign, a dropped
+term, a transposed weight -- fails it: exact rational slab tests, closed-form segment lists of axis-aligned rays,
+chord lengths of full tilings, torch.grid_sample / nn.Linear / index_add_ in float64 with autograd, constant- and
+affine-field line integrals, and central finite differences.
+"""
+import math
+from fractions import Fraction as Fr
+
+import numpy as np
+import pytest
+import torch
+
+F32 = np.float32
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# tree / active set
+# ---------------------------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("depth,count", [(0, 1), (1, 9), (2, 73), (3, 585), (5, 37449)])
+def test_tree_node_counts(oracle, depth, count):
+    assert oracle.build_tree(depth)["pos_min"].shape[0] == count
+
+
+def test_tree_boxes_are_dyadic_octants(oracle):
+    t = oracle.build_tree(3)
+    pmin, pmax, depth, ch, par = t["pos_min"], t["pos_max"], t["depth"], t["children"], t["parent"]
+    assert np.all(pmin[0] == -1) and np.all(pmax[0] == 1) and par[0] == -1
+    # edge length 2^(1-depth), every box inside [-1,1]^3 on the dyadic lattice of its depth
+    edge = 2.0 ** (1 - depth)
+    assert np.allclose(pmax - pmin, edge[:, None], rtol=0, atol=0)
+    lattice = (pmin + 1) / edge[:, None]
+    assert np.all(lattice == np.round(lattice))
+    for n in range(73):  # depth < 3: children are the 8 octants, bit k of i = upper half on axis k
+        c = (pmin[n] + pmax[n]) / 2
+        for i in range(8):
+            k = ch[n, i]
+            assert k == 8 * n + 1 + i and par[k] == n
+            for ax in range(3):
+                up = (i >> ax) & 1
+                assert pmin[k, ax] == (c[ax] if up else pmin[n, ax])
+                assert pmax[k, ax] == (pmax[n, ax] if up else c[ax])
+    assert np.all(ch[73:] == -1)
+    assert np.allclose(t["scale"], depth / 3 * 2 - 1)
+
+
+def test_update_active_matches_brute_force(oracle):
+    rng = np.random.default_rng(0)
+    act = (rng.random(585) < 0.3).astype(np.int32)
+    ids, prefix = oracle.update_active(act)
+    assert np.array_equal(ids, np.nonzero(act)[0])
+    exp = np.full(585, -1)
+    exp[ids] = np.arange(len(ids))
+    assert np.array_equal(prefix, exp)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# slab test against exact rational arithmetic
+# ---------------------------------------------------------------------------------------------------------------------
+def exact_interval(bmin, bmax, o, d):
+    tn, tf = None, None
+    for i in range(3):
+        if d[i] == 0:
+            if not (Fr(float(bmin[i])) <= Fr(float(o[i])) < Fr(float(bmax[i]))):
+                return None
+            continue
+        t1 = (Fr(float(bmin[i])) - Fr(float(o[i]))) / Fr(float(d[i]))
+        t2 = (Fr(float(bmax[i])) - Fr(float(o[i]))) / Fr(float(d[i]))
+        if t1 > t2:
+            t1, t2 = t2, t1
+        tn = t1 if tn is None or t1 > tn else tn
+        tf = t2 if tf is None or t2 < tf else tf
+    if tn is None:
+        return (Fr(-236436436), Fr(43637575))
+    if tn > tf or tf < 0:
+        return None
+    return tn, tf
+
+
+def test_intersect_matches_exact_rationals(oracle):
+    rng = np.random.default_rng(1)
+    checked = 0
+    for _ in range(3000):
+        lo = rng.uniform(-1, 0.8, 3).astype(F32)
+        hi = (lo + rng.uniform(0.05, 0.6, 3)).astype(F32)
+        o = rng.uniform(-3, 3, 3).astype(F32)
+        d = rng.normal(size=3)
+        if rng.random() < 0.2:
+            d[rng.integers(3)] = 0
+        d = (d / np.linalg.norm(d)).astype(F32)
+        hit, tmin, tmax = oracle.intersect(lo, hi, o, d)
+        ex = exact_interval(lo, hi, o, d)
+        if ex is None:
+            # a miss by a clear margin must be a miss
+            continue_ok = True
+            if hit:
+                assert tmax - tmin < 1e-4, "oracle reports a hit where the exact slab test misses"
+            checked += continue_ok
+            continue
+        tn, tf = float(ex[0]), float(ex[1])
+        if tf - tn > 1e-4 and tf > 1e-4:
+            assert hit
+            assert abs(tmin - tn) <= 4e-6 * max(1, abs(tn))
+            assert abs(tmax - tf) <= 4e-6 * max(1, abs(tf))
+        checked += 1
+    assert checked > 2500
+
+
+def test_intersect_parallel_axis_half_open(oracle):
+    lo = np.array([-0.5, -0.5, -0.5], F32)
+    hi = np.array([0.0, 0.0, 0.0], F32)
+    d = np.array([1, 0, 0], F32)
+    # parallel axes: inside iff min <= o < max (HyperTree.h:51)
+    assert oracle.intersect(lo, hi, np.array([-3, -0.5, -0.25], F32), d)[0]
+    assert not oracle.intersect(lo, hi, np.array([-3, 0.0, -0.25], F32), d)[0]
+    hit, t0, t1 = oracle.intersect(lo, hi, np.array([-3, -0.25, -0.25], F32), d)
+    assert hit and t0 == 2.5 and t1 == 3.0
+    # box behind the origin
+    assert not oracle.intersect(lo, hi, np.array([3, -0.25, -0.25], F32), d)[0]
+    # origin inside: tmin negative
+    hit, t0, t1 = oracle.intersect(lo, hi, np.array([-0.25, -0.25, -0.25], F32), d)
+    assert hit and t0 == -0.25 and t1 == 0.25
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# ray sampling
+# ---------------------------------------------------------------------------------------------------------------------
+def leaf_of(p, depth):
+    """Breadth-first id of the depth-`depth` node containing point p (descending octant bits)."""
+    n = 0
+    lo, hi = np.full(3, -1.0), np.full(3, 1.0)
+    for _ in range(depth):
+        c = (lo + hi) / 2
+        i = sum(1 << k for k in range(3) if p[k] >= c[k])
+        for k in range(3):
+            if p[k] >= c[k]:
+                lo[k] = c[k]
+            else:
+                hi[k] = c[k]
+        n = 8 * n + 1 + i
+    return n
+
+
+def full_tiling(oracle, depth):
+    t = oracle.build_tree(depth)
+    ids, prefix = oracle.update_active((t["depth"] == depth).astype(np.int32))
+    return t, ids, prefix
+
+
+def test_axis_aligned_ray_closed_form(oracle):
+    t, ids, _ = full_tiling(oracle, 2)
+    o = np.array([[-3.0, -0.6, 0.3]], F32)
+    d = np.array([[1.0, 0.0, 0.0]], F32)
+    r = oracle.raysample(t, ids, o, d, 32)
+    # four cells of edge 0.5 along x: t in [2,2.5], [2.5,3], [3,3.5], [3.5,4]
+    assert list(r["seg_tmin"]) == [2.0, 2.5, 3.0, 3.5]
+    assert list(r["seg_tmax"]) == [2.5, 3.0, 3.5, 4.0]
+    n = math.ceil(0.5 / math.sqrt(3 * 0.25) * 32)  # 19
+    assert list(r["seg_count"]) == [n] * 4
+    exp_nodes = [leaf_of(np.array([x, -0.6, 0.3]), 2) for x in (-0.75, -0.25, 0.25, 0.75)]
+    assert list(r["seg_node"]) == exp_nodes
+    # midpoint samples and telescoping weights
+    tt = r["t"].reshape(4, n).astype(np.float64)
+    exp_t = np.array([2.0, 2.5, 3.0, 3.5])[:, None] + (np.arange(n) + 0.5) * 0.5 / n
+    assert np.allclose(tt, exp_t, atol=2e-6)
+    ww = r["weight"].reshape(4, n)
+    assert np.allclose(ww.sum(1), 0.5, atol=2e-6)
+    assert np.allclose(r["coord"][:, 0], -3 + r["t"], atol=1e-6) and np.all(r["coord"][:, 1] == F32(-0.6))
+    assert np.array_equal(r["local_index"], np.arange(4 * n))
+
+
+def test_ray_on_cell_face_belongs_to_upper_cell(oracle):
+    t, ids, _ = full_tiling(oracle, 2)
+    o = np.array([[-3.0, -0.5, 0.0]], F32)
+    d = np.array([[1.0, 0.0, 0.0]], F32)
+    r = oracle.raysample(t, ids, o, d, 8)
+    assert list(r["seg_node"]) == [leaf_of(np.array([x, -0.5, 0.0]), 2) for x in (-0.75, -0.25, 0.25, 0.75)]
+
+
+def cube_chord(o, d):
+    ex = exact_interval(np.array([-1, -1, -1], F32), np.array([1, 1, 1], F32), o, d)
+    if ex is None:
+        return 0.0
+    return float(ex[1] - max(ex[0], ex[0]))
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3])
+def test_segment_lengths_sum_to_chord(oracle, depth):
+    t, ids, _ = full_tiling(oracle, depth)
+    rng = np.random.default_rng(depth)
+    o = np.tile(np.array([0.3, -4.0, 0.2], F32), (400, 1))
+    d = rng.normal(size=(400, 3))
+    d[:, 1] = np.abs(d[:, 1]) * 3
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(F32)
+    r = oracle.raysample(t, ids, o, d, 16)
+    lens = (r["seg_tmax"].astype(np.float64) - r["seg_tmin"])
+    per_ray = np.zeros(400)
+    np.add.at(per_ray, r["seg_ray"], lens)
+    nseg = np.diff(r["ray_seg_offset"])
+    for i in range(400):
+        ch = cube_chord(o[i], d[i])
+        # dropped slivers are < 1e-5 each; float rounding of t is ~1e-6
+        assert abs(per_ray[i] - ch) <= 1e-5 * (nseg[i] + 2) + 4e-6 * ch, (i, per_ray[i], ch)
+    # weights telescope to the segment length; t increases within a segment; canonical order
+    for k in range(len(r["seg_node"])):
+        a, n = r["seg_sample_offset"][k], r["seg_count"][k]
+        assert abs(r["weight"][a:a + n].astype(np.float64).sum() - lens[k]) <= 2e-6 * max(1, lens[k])
+        assert np.all(np.diff(r["t"][a:a + n]) > 0)
+    for i in range(400):
+        a, b = r["ray_seg_offset"][i], r["ray_seg_offset"][i + 1]
+        key = list(zip(r["seg_tmin"][a:b], r["seg_node"][a:b]))
+        assert key == sorted(key)
+
+
+def test_counts_bounded_by_max_spn(oracle):
+    t, ids, _ = full_tiling(oracle, 2)
+    rng = np.random.default_rng(3)
+    o = rng.uniform(-0.9, 0.9, (300, 3)).astype(F32)
+    d = rng.normal(size=(300, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(F32)
+    r = oracle.raysample(t, ids, o, d, 32, emit=False)
+    # a chord of a cube is at most its diagonal: 1 <= n <= max_spn (+1 if rounding pushes len/diag above 1)
+    assert r["seg_count"].min() >= 1 and r["seg_count"].max() <= 33
+
+
+def test_jitter_stays_in_cell_and_is_deterministic(oracle):
+    t, ids, _ = full_tiling(oracle, 2)
+    o = np.array([[-3.0, -0.6, 0.3]], F32)
+    d = np.array([[1.0, 0.0, 0.0]], F32)
+    r1 = oracle.raysample(t, ids, o, d, 32, jitter=True, seed=7)
+    r2 = oracle.raysample(t, ids, o, d, 32, jitter=True, seed=7)
+    r3 = oracle.raysample(t, ids, o, d, 32, jitter=True, seed=8)
+    assert np.array_equal(r1["t"], r2["t"]) and not np.array_equal(r1["t"], r3["t"])
+    n = r1["seg_count"][0]
+    h = 0.5 / n
+    tt = r1["t"][:n].astype(np.float64)
+    cell = np.floor((tt - 2.0) / h)
+    assert np.array_equal(cell, np.arange(n))
+    a = np.array([oracle.jitter(7, 0, 0, j) for j in range(2000)])
+    assert a.min() >= 0 and a.max() < 1 and abs(a.mean() - 0.5) < 0.03
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# local coordinates, grid sample, decoder, integration
+# ---------------------------------------------------------------------------------------------------------------------
+def test_local_samples_faces_and_center(oracle):
+    t = oracle.build_tree(2)
+    n = 20
+    mn, mx = t["pos_min"][n], t["pos_max"][n]
+    pts = np.array([mn, mx, (mn + mx) / 2, [mn[0], mx[1], (mn[2] + mx[2]) / 2]], F32)
+    out = oracle.local_samples(t, pts, np.full(4, n))
+    assert np.array_equal(out[0], [-1, -1, -1]) and np.array_equal(out[1], [1, 1, 1])
+    assert np.array_equal(out[2], [0, 0, 0]) and np.array_equal(out[3], [-1, 1, 0])
+
+
+def torch_grid_sample(grid, index, uv):
+    dup = grid[index]
+    g = uv.reshape(-1, 1, 1, 1, 3)
+    return torch.nn.functional.grid_sample(dup, g, mode="bilinear", padding_mode="border",
+                                           align_corners=True).reshape(index.shape[0], -1)
+
+
+@pytest.mark.parametrize("channels_last", [False, True])
+def test_grid_sample_matches_torch(oracle, channels_last):
+    rng = np.random.default_rng(4)
+    N, C, G = 5, 8, 7
+    grid = rng.normal(size=(N, C, G, G + 1, G + 2))
+    idx = rng.integers(0, N, 300)
+    uv = rng.uniform(-1, 1, (300, 3))
+    uv[:10] = rng.choice([-1.0, 1.0], (10, 3))  # corners return grid values exactly
+    g_np = np.ascontiguousarray(grid.transpose(0, 2, 3, 4, 1)).transpose(0, 4, 1, 2, 3) if channels_last else grid
+    out = oracle.grid_sample_fwd(g_np, idx, uv)
+    tg = torch.tensor(grid, dtype=torch.float64, requires_grad=True)
+    tuv = torch.tensor(uv, dtype=torch.float64, requires_grad=True)
+    ref = torch_grid_sample(tg, torch.tensor(idx), tuv)
+    assert np.allclose(out, ref.detach().numpy(), rtol=1e-12, atol=1e-12)
+    go = rng.normal(size=out.shape)
+    (ref * torch.tensor(go)).sum().backward()
+    gg, guv = oracle.grid_sample_bwd(g_np, idx, uv, go)
+    assert np.allclose(np.asarray(gg), tg.grad.numpy(), rtol=1e-10, atol=1e-12)
+    inner = np.all(np.abs(uv) < 1 - 1e-9, axis=1)  # torch clips the border, the reference clamps indices
+    assert np.allclose(guv[inner], tuv.grad.numpy()[inner], rtol=1e-9, atol=1e-10)
+
+
+def test_grid_sample_affine_field_is_exact(oracle):
+    G = 9
+    z, y, x = np.meshgrid(np.arange(G), np.arange(G), np.arange(G), indexing="ij")
+    f = (0.3 * x - 0.7 * y + 1.1 * z + 0.2).astype(np.float64)
+    grid = f[None, None]
+    uv = np.random.default_rng(5).uniform(-1, 1, (200, 3))
+    p = (uv + 1) / 2 * (G - 1)
+    exp = 0.3 * p[:, 0] - 0.7 * p[:, 1] + 1.1 * p[:, 2] + 0.2
+    out = oracle.grid_sample_fwd(grid, np.zeros(200, np.int64), uv)[:, 0]
+    assert np.allclose(out, exp, rtol=1e-12, atol=1e-12)
+
+
+def torch_mlp(params, d_in, hidden, layers, d_out, act, last_act, beta):
+    dims = [d_in] + [hidden] * (layers + 1) + [d_out]
+    mods, off = [], 0
+    for i in range(len(dims) - 1):
+        lin = torch.nn.Linear(dims[i], dims[i + 1]).double()
+        nw = dims[i] * dims[i + 1]
+        with torch.no_grad():
+            lin.weight.copy_(torch.tensor(params[off:off + nw]).view(dims[i + 1], dims[i]))
+            off += nw
+            lin.bias.copy_(torch.tensor(params[off:off + dims[i + 1]]))
+            off += dims[i + 1]
+        mods.append(lin)
+        if i < len(dims) - 2:
+            mods.append({"relu": torch.nn.ReLU(), "silu": torch.nn.SiLU(), "id": torch.nn.Identity()}[act])
+    last = {"softplus": torch.nn.Softplus(beta=beta), "relu": torch.nn.ReLU(), "id": torch.nn.Identity(),
+            "abs": None}[last_act]
+    seq = torch.nn.Sequential(*mods)
+    return (lambda x: seq(x).abs()) if last is None else (lambda x: last(seq(x))), seq
+
+
+@pytest.mark.parametrize("act,last_act,hidden,layers", [("relu", "softplus", 32, 1), ("silu", "softplus", 64, 1),
+                                                        ("relu", "relu", 16, 0), ("silu", "abs", 32, 2)])
+def test_decoder_matches_torch(oracle, act, last_act, hidden, layers):
+    rng = np.random.default_rng(6)
+    d_in, d_out, beta = 8, 1, 2.0
+    P = oracle.decoder_num_params(d_in, hidden, layers, d_out)
+    params = rng.normal(size=P) * 0.4
+    x = rng.normal(size=(100, d_in))
+    y = oracle.decoder_fwd(x, params, d_in, hidden, layers, d_out, act, last_act, beta)
+    f, seq = torch_mlp(params, d_in, hidden, layers, d_out, act, last_act, beta)
+    tx = torch.tensor(x, requires_grad=True)
+    ty = f(tx)
+    assert np.allclose(y, ty.detach().numpy(), rtol=1e-12, atol=1e-13)
+    gy = rng.normal(size=y.shape)
+    (ty * torch.tensor(gy)).sum().backward()
+    gp, gx = oracle.decoder_bwd(x, params, gy, d_in, hidden, layers, d_out, act, last_act, beta)
+    tp = torch.cat([p.grad.reshape(-1) for p in seq.parameters()]).numpy()
+    assert np.allclose(gp, tp, rtol=1e-10, atol=1e-12)
+    assert np.allclose(gx, tx.grad.numpy(), rtol=1e-10, atol=1e-12)
+
+
+def test_softplus_threshold_is_linear(oracle):
+    P = oracle.decoder_num_params(1, 16, 0, 1)
+    params = np.zeros(P)
+    params[-1] = 15.0  # output bias; beta * x = 30 > 20
+    y = oracle.decoder_fwd(np.zeros((1, 1)), params, 1, 16, 0, 1, "relu", "softplus", 2.0)
+    assert y[0, 0] == 15.0
+
+
+def test_integrate_matches_index_add(oracle):
+    rng = np.random.default_rng(7)
+    v = rng.normal(size=(500, 2)).astype(F32)
+    w = rng.uniform(0, 1, 500).astype(F32)
+    ri = rng.integers(0, 40, 500)
+    out = oracle.integrate_fwd(v, w, ri, 40)
+    ref = torch.zeros(2, 40, dtype=torch.float64).index_add_(1, torch.tensor(ri),
+                                                                torch.tensor(v * w[:, None], dtype=torch.float64).T)
+    assert np.allclose(out, ref.numpy(), rtol=1e-5, atol=1e-6)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# composed projector
+# ---------------------------------------------------------------------------------------------------------------------
+def _rays(n, seed):
+    rng = np.random.default_rng(seed)
+    th = rng.uniform(0, 2 * np.pi, n)
+    src = np.stack([np.sin(-th), -np.cos(-th), np.zeros(n)], 1) * 4
+    tgt = rng.uniform(-0.9, 0.9, (n, 3))
+    d = tgt - src
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return src.astype(F32), d.astype(F32)
+
+
+def _linear_mlp(C, hidden, kappa=None, take_channel=None):
+    """Decoder with id activations: a constant (kappa) or a pass-through of one grid channel."""
+    P = (C * hidden + hidden) + (hidden * hidden + hidden) + (hidden + 1)
+    p = np.zeros(P, F32)
+    if kappa is not None:
+        p[-1] = kappa
+    else:
+        p[take_channel] = 1.0                          # W0[0][channel]
+        p[C * hidden + hidden + 0] = 1.0               # W1[0][0]
+        p[C * hidden + hidden + hidden * hidden + hidden + 0] = 1.0  # Wout[0][0]
+    return dict(params=p, **{"in": C}, hidden=hidden, layers=1, out=1, act="id", last_act="id", beta=2.0)
+
+
+def test_projector_constant_field_gives_kappa_times_chord(oracle):
+    t, ids, prefix = full_tiling(oracle, 2)
+    o, d = _rays(200, 8)
+    segs = oracle.raysample(t, ids, o, d, 32, emit=False)
+    grid = np.random.default_rng(9).normal(size=(64, 8, 5, 5, 5)).astype(F32)
+    mlp = _linear_mlp(8, 16, kappa=0.75)
+    for prec in (0, 1):
+        proj = oracle.projector_fwd(t, prefix, o, d, segs, grid, mlp, prec=prec)
+        chords = np.array([cube_chord(o[i], d[i]) for i in range(200)])
+        nseg = np.diff(segs["ray_seg_offset"])
+        assert np.all(np.abs(proj - 0.75 * chords) <= 0.75 * (1e-5 * (nseg + 2)) + 1e-5 * chords)
+
+
+def test_projector_affine_field_gives_exact_line_integral(oracle):
+    depth, G = 2, 5
+    t, ids, prefix = full_tiling(oracle, depth)
+    # dyadic coefficients: the grid values a.p + b at dyadic lattice points are exact in float32
+    a = np.array([0.375, -0.25, 0.875])
+    b = 0.25
+    grid = np.zeros((len(ids), 8, G, G, G), F32)
+    lin = np.linspace(0, 1, G)
+    for slot, n in enumerate(ids):
+        mn, mx = t["pos_min"][n].astype(np.float64), t["pos_max"][n].astype(np.float64)
+        zz, yy, xx = np.meshgrid(lin, lin, lin, indexing="ij")
+        px = mn[0] + xx * (mx[0] - mn[0])
+        py = mn[1] + yy * (mx[1] - mn[1])
+        pz = mn[2] + zz * (mx[2] - mn[2])
+        grid[slot, 3] = a[0] * px + a[1] * py + a[2] * pz + b
+    o, d = _rays(150, 10)
+    segs = oracle.raysample(t, ids, o, d, 16, emit=False)
+    mlp = _linear_mlp(8, 16, take_channel=3)
+    proj = oracle.projector_fwd(t, prefix, o, d, segs, grid, mlp, prec=1)
+    exp = np.zeros(150)
+    for k in range(len(segs["seg_node"])):
+        r = segs["seg_ray"][k]
+        n = segs["seg_node"][k]
+        # the FP64 oracle rederives the segment ends in double: compare with the exact interval of the node box
+        ex = exact_interval(t["pos_min"][n], t["pos_max"][n], o[r], d[r])
+        t0, t1 = float(ex[0]), float(ex[1])
+        oo, dd = o[r].astype(np.float64), d[r].astype(np.float64)
+        exp[r] += (a @ oo + b) * (t1 - t0) + (a @ dd) * (t1 * t1 - t0 * t0) / 2
+    assert np.allclose(proj, exp, rtol=1e-9, atol=1e-9)
+
+
+def test_projector_fp32_matches_fp64(oracle):
+    t, ids, prefix = full_tiling(oracle, 2)
+    o, d = _rays(300, 11)
+    segs = oracle.raysample(t, ids, o, d, 32, emit=False)
+    rng = np.random.default_rng(12)
+    grid = (rng.normal(size=(64, 8, 11, 11, 11)) * 0.1).astype(F32)
+    P = oracle.decoder_num_params(8, 32, 1, 1)
+    mlp = dict(params=(rng.normal(size=P) * 0.3).astype(F32), **{"in": 8}, hidden=32, layers=1, out=1, act="relu",
+               last_act="softplus", beta=2.0)
+    p32 = oracle.projector_fwd(t, prefix, o, d, segs, grid, mlp, prec=0)
+    p64 = oracle.projector_fwd(t, prefix, o, d, segs, grid, mlp, prec=1)
+    assert np.linalg.norm(p32 - p64) / np.linalg.norm(p64) < 1e-5
+
+
+def test_projector_gradient_matches_central_differences(oracle):
+    t, ids, prefix = full_tiling(oracle, 1)
+    o, d = _rays(40, 13)
+    segs = oracle.raysample(t, ids, o, d, 8, emit=False)
+    rng = np.random.default_rng(14)
+    grid = (rng.normal(size=(8, 4, 5, 5, 5)) * 0.3).astype(F32)
+    P = oracle.decoder_num_params(4, 16, 1, 1)
+    params = (rng.normal(size=P) * 0.4).astype(F32)
+    mlp = dict(params=params, **{"in": 4}, hidden=16, layers=1, out=1, act="silu", last_act="softplus", beta=2.0)
+    gproj = rng.normal(size=40)
+    gg, gp = oracle.projector_bwd(t, prefix, o, d, segs, grid, mlp, gproj, prec=1)
+
+    def loss(gr, pr):
+        m = dict(mlp, params=pr)
+        return float(gproj @ oracle.projector_fwd(t, prefix, o, d, segs, gr, m, prec=1))
+
+    eps = 1e-3
+    flat = grid.reshape(-1)
+    for i in rng.choice(flat.size, 24, replace=False):
+        gpl, gmi = flat.copy(), flat.copy()
+        gpl[i] += eps
+        gmi[i] -= eps
+        # float32 grid entries: use the actually representable perturbation
+        h = float(gpl[i]) - float(gmi[i])
+        fd = (loss(gpl.reshape(grid.shape), params) - loss(gmi.reshape(grid.shape), params)) / h
+        assert abs(fd - gg.reshape(-1)[i]) <= 2e-3 * max(1e-2, abs(fd)), (i, fd, gg.reshape(-1)[i])
+    for i in rng.choice(P, 12, replace=False):
+        ppl, pmi = params.copy(), params.copy()
+        ppl[i] += eps
+        pmi[i] -= eps
+        h = float(ppl[i]) - float(pmi[i])
+        fd = (loss(grid, ppl) - loss(grid, pmi)) / h
+        assert abs(fd - gp[i]) <= 2e-3 * max(1e-2, abs(fd)), (i, fd, gp[i])
