@@ -1,0 +1,210 @@
+#!/usr/bin/env python
+"""Projector benchmark: fwd+bwd rays/s on one MI355X.
+
+One step = one fwd+bwd pass of the projector over one batch of 2^18 synthetic cone-beam rays of the full-size
+configuration (depth-5 octree adapted to a 100-ellipsoid phantom, 4096 leaves, 17^3 x 8 grids, decoder
+8->64->64->1, 256 views x 512^2 detector, max 64 samples per node): ray sampling, gather, decoder, line integral,
+FP64 MSE against the phantom's closed-form projections, and the backward to grid and decoder gradients.
+Inputs (rays, targets, grid, decoder) are resident in HBM before the timed region.
+
+Prints one JSON line (rank 0). Besides the throughput it carries the roofline of the dominant kernel (timed with
+HIP events on the stream it runs on) and a CPU baseline: the oracle (the reference's CPU path restated in C++)
+timed on a bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--path", default="fused", choices=["fused", "unfused"])
+    ap.add_argument("--batches", type=int, default=4, help="distinct resident ray batches cycled by the steps")
+    ap.add_argument("--cpu-rays", type=int, default=2048, help="rays in the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it", "n_gpus": a.gpus}))
+        return 2
+    import torch
+
+    from neat_amd import ops, pipeline, workloads as W
+
+    cfg = W.C2
+    dev = "cuda"
+    scene = W.make_scene(cfg, seed=a.seed)
+    ds = pipeline.to_device(scene, cfg, dev)
+    batches = []
+    for b in range(a.batches):
+        o, d = W.batch_rays(cfg, cfg.batch, seed=a.seed + 100 + b)
+        tgt = scene["phantom"].line_integrals(o, d).astype(np.float32)
+        batches.append((torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev), torch.from_numpy(tgt).to(dev)))
+    torch.cuda.synchronize()
+
+    stage_names = (["sample", "plan", "proj_fwd", "loss", "proj_bwd"] if a.path == "fused" else
+                   ["sample", "gs_fwd", "dec_fwd", "integrate", "dec_bwd", "gs_bwd"])
+    events = {k: [] for k in stage_names}
+
+    def fused_step(i, timed):
+        o, d, tgt = batches[i % len(batches)]
+        ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for k in stage_names}
+        rec = (lambda k, j: ev[k][j].record()) if timed else (lambda k, j: None)
+        rec("sample", 0)
+        segs = pipeline.sample(ds, o, d, jitter=True, seed=i, emit=False)
+        rec("sample", 1)
+        rec("plan", 0)
+        pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, o, d, segs, ds.grid, ds.params, ds.mlp, True, i)
+        rec("plan", 1)
+        rec("proj_fwd", 0)
+        proj = pj.forward()
+        rec("proj_fwd", 1)
+        rec("loss", 0)
+        loss, gproj = ops.mse_fp64(proj, tgt)
+        rec("loss", 1)
+        rec("proj_bwd", 0)
+        ggrid, gparams = pj.backward(gproj)
+        rec("proj_bwd", 1)
+        if timed:
+            for k in stage_names:
+                events[k].append(ev[k])
+        return pj.S
+
+    def step(i, timed):
+        if a.path == "fused":
+            return fused_step(i, timed)
+        o, d, tgt = batches[i % len(batches)]
+        R = o.shape[0]
+        ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for k in stage_names}
+        rec = (lambda k, j: ev[k][j].record()) if timed else (lambda k, j: None)
+        rec("sample", 0)
+        smp = pipeline.sample(ds, o, d, jitter=True, seed=i)
+        rec("sample", 1)
+        uv = ops.local_samples(smp["coord"], smp["node_id"], ds.pos_min, ds.pos_max)
+        slot = ops.node_to_slot(smp["node_id"], ds.prefix)
+        rec("gs_fwd", 0)
+        feat = ops.gs3d_fwd(ds.grid, slot, uv)
+        rec("gs_fwd", 1)
+        rec("dec_fwd", 0)
+        y = ops.decoder_fwd(feat, ds.params, *pipeline._mlp_args(ds))
+        rec("dec_fwd", 1)
+        rec("integrate", 0)
+        proj = ops.integrate_fwd(y, smp["weight"], smp["ray_index"], R, smp["ray_sample_offset"])
+        loss, gproj = ops.mse_fp64(proj.reshape(-1), tgt)
+        gy = ops.integrate_bwd(gproj.reshape(1, R), smp["weight"], smp["ray_index"])
+        rec("integrate", 1)
+        rec("dec_bwd", 0)
+        gparams, gfeat = ops.decoder_bwd(feat, ds.params, gy, *pipeline._mlp_args(ds))
+        rec("dec_bwd", 1)
+        rec("gs_bwd", 0)
+        ggrid, _ = ops.gs3d_bwd(ds.grid, slot, uv, gfeat, want_uv=False)
+        rec("gs_bwd", 1)
+        if timed:
+            for k in stage_names:
+                events[k].append(ev[k])
+        return int(smp["ray_sample_offset"][R].item())
+
+    for i in range(a.warmup):
+        step(i, False)
+    torch.cuda.synchronize()
+    samples = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        samples.append(step(a.warmup + i, True))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / a.steps
+    rays_per_s = cfg.batch / (ms_per_step / 1e3)
+    mean_samples = float(np.mean(samples))
+
+    stage_ms = {k: float(np.mean([s.elapsed_time(e) for s, e in events[k]])) for k in stage_names}
+    dominant = max(stage_ms, key=stage_ms.get)
+    C, H = cfg.channels, cfg.hidden
+    fwd_flop = 2 * (C * H + H * H + H * 1)  # per sample, 2 FLOP per MAC
+    # algorithmic work per launch of each stage (per-sample figure x samples of the batch)
+    work = {
+        "proj_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS),
+        "proj_bwd": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS),
+        "plan": ("hbm", 16 * mean_samples / 27, "GB/s", HBM_PEAK_GBS),
+        "loss": ("hbm", 12 * cfg.batch, "GB/s", HBM_PEAK_GBS),
+        "dec_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS),
+        "dec_bwd": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS),
+        "gs_fwd": ("hbm", (12 + 8 + 4 * C) * mean_samples, "GB/s", HBM_PEAK_GBS),
+        "gs_bwd": ("hbm", (12 + 8 + 4 * C) * mean_samples + 2 * ds.grid.numel() * 4, "GB/s", HBM_PEAK_GBS),
+        "sample": ("hbm", 44 * mean_samples + 32 * cfg.batch, "GB/s", HBM_PEAK_GBS),
+        "integrate": ("hbm", (4 + 4 + 8) * mean_samples * 2 + 16 * cfg.batch, "GB/s", HBM_PEAK_GBS),
+    }
+    bound, amount, unit, peak = work[dominant]
+    secs = stage_ms[dominant] / 1e3
+    achieved = amount / secs / (1e12 if unit == "TFLOP/s" else 1e9)
+    roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                "frac": round(achieved / peak, 5), "traffic": None, "kernel": dominant,
+                "kernel_ms": round(stage_ms[dominant], 4)}
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(scene, cfg, a.cpu_rays, a.seed)
+
+    out = {
+        "metric": "fwd+bwd rays/s at 1 GPU, BF16+proposal vs plain FP32, at <=1e-3 rel err vs FP64",
+        "value": round(rays_per_s, 1),
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "C2 plain FP32 projector fwd+bwd: depth-5 octree, 4096 adaptive leaves, 17^3x8 grids, "
+                               "decoder 8->64->64->1, 256x512^2 cone beam, batch 2^18 rays, max 64 samples/node",
+                   "path": a.path, "batch_rays": cfg.batch, "samples_per_batch": round(mean_samples),
+                   "samples_per_s": round(mean_samples / (ms_per_step / 1e3), 1),
+                   "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline(scene, cfg, num_rays, seed):
+    """The oracle's FP32 restatement of the reference path (sampling by brute force over the active leaves, then
+    the composed forward and backward), single-threaded like the reference's CPU loops, on a bounded ray sample."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from neat_amd import workloads as W
+    from oracle import oracle as O
+    o, d = W.batch_rays(cfg, num_rays, seed=seed + 999)
+    tgt = scene["phantom"].line_integrals(o, d)
+    t0 = time.perf_counter()
+    segs = O.raysample(scene["tree"], scene["active_ids"], o, d, cfg.max_spn, emit=False, nthreads=1)
+    proj = O.projector_fwd(scene["tree"], scene["prefix"], o, d, segs, scene["grid"], dict(scene["mlp"],
+                           params=scene["params"]), jitter=True, seed=seed, prec=0, nthreads=1)
+    gproj = 2 * (proj - tgt) / num_rays
+    O.projector_bwd(scene["tree"], scene["prefix"], o, d, segs, scene["grid"], dict(scene["mlp"],
+                    params=scene["params"]), gproj, jitter=True, seed=seed, prec=0, nthreads=1)
+    dt = time.perf_counter() - t0
+    return {"value": round(num_rays / dt, 2), "unit": "rays/s", "cores": 1, "kind": "port",
+            "sample": f"{num_rays} rays of the same workload (sampling + fwd + bwd), single-threaded oracle, "
+                      f"{dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -185,6 +185,68 @@ size_t neat_mse_workspace_bytes(int64_t n);
 int neat_mse_fp64(neat_stream_t stream, const float* pred, const float* target, int64_t n, float grad_scale,
                   double* out_loss, float* grad_pred, void* workspace, size_t workspace_bytes);
 
+/* ---- fused projector -------------------------------------------------------------------------------------------------
+ * One call per direction for the whole chain of HierarchicalNeuralGeometry::ComputeImage (geometry.h:170,
+ * geometry.cpp:347-389) over GeometryExEx::SampleVolumeIndirect (geometry_ex_ex.cpp:161-180): sample positions are
+ * recomputed from the canonical segment list (neat_raysample_segments), gathered from the node grids, decoded,
+ * integrated per ray; the backward returns grid and decoder gradients. Covered configuration: grid cubic and
+ * channels-last contiguous, in = channels in {4, 8}, hidden in {32, 64}, one hidden layer, out = 1, relu or silu
+ * hidden activation, softplus output; anything else returns NEAT_E_UNSUPPORTED (use the unfused entry points). */
+typedef struct neat_projector_args
+{
+    /* octree */
+    const float* node_pos_min;
+    const float* node_pos_max;
+    const int64_t* node_active_prefix_sum; /* node -> grid slot, -1 when inactive */
+    int32_t num_nodes;
+    /* rays and their canonical segments */
+    const float* ray_origin;
+    const float* ray_direction;
+    int64_t num_rays;
+    const int64_t* ray_seg_offset; /* [num_rays + 1] */
+    int64_t num_segments;
+    const int32_t* seg_node;
+    const int32_t* seg_ray;
+    const int32_t* seg_count;
+    const float* seg_tmin;
+    const float* seg_tmax;
+    int64_t num_samples; /* total of seg_count */
+    int32_t jitter;
+    uint64_t seed;
+    /* feature grid [num_slots, channels, G, G, G] with element strides */
+    const float* grid;
+    int64_t num_slots;
+    int32_t channels;
+    int32_t grid_size;
+    int64_t grid_strides[5];
+    /* decoder (packed parameters) */
+    int32_t in_features;
+    int32_t hidden;
+    int32_t hidden_layers;
+    int32_t out_features;
+    const float* params;
+    int32_t act;
+    int32_t last_act;
+    float softplus_beta;
+} neat_projector_args;
+
+/* Workspace for plan + forward + backward of these arguments (depends on the counts, not on the data). */
+size_t neat_projector_workspace_bytes(const neat_projector_args* args);
+
+/* Sorts the segments by grid slot and cuts the samples into node-local chunks. Must run (on the same workspace)
+ * after the segments change and before neat_projector_fwd / _bwd. */
+int neat_projector_plan(neat_stream_t stream, const neat_projector_args* args, void* workspace, size_t workspace_bytes);
+
+/* proj [num_rays] = per-ray line integral of softplus(decoder(trilinear(grid))) (compensated != 0: Neumaier sum of
+ * the per-segment partials). */
+int neat_projector_fwd(neat_stream_t stream, const neat_projector_args* args, int32_t compensated, float* proj,
+                       void* workspace, size_t workspace_bytes);
+
+/* Gradients of sum_r grad_proj[r] * proj[r]: grad_grid (same layout as the grid, fully written) and grad_params
+ * (packed, written). Samples are recomputed; no activations are kept from the forward. */
+int neat_projector_bwd(neat_stream_t stream, const neat_projector_args* args, const float* grad_proj,
+                       float* grad_grid, float* grad_params, void* workspace, size_t workspace_bytes);
+
 #ifdef __cplusplus
 }
 #endif

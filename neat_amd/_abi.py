@@ -51,6 +51,29 @@ SIGNATURES = {
     "neat_mse_fp64": (ctypes.c_int, [_P, _P, _P, _I64, _F, _P, _P, _P, _SZ]),
 }
 
+
+
+class ProjectorArgs(ctypes.Structure):
+    """Mirror of neat_projector_args (include/neat_amd.h)."""
+    _fields_ = [
+        ("node_pos_min", _P), ("node_pos_max", _P), ("node_active_prefix_sum", _P), ("num_nodes", _I32),
+        ("ray_origin", _P), ("ray_direction", _P), ("num_rays", _I64), ("ray_seg_offset", _P),
+        ("num_segments", _I64), ("seg_node", _P), ("seg_ray", _P), ("seg_count", _P), ("seg_tmin", _P),
+        ("seg_tmax", _P), ("num_samples", _I64), ("jitter", _I32), ("seed", _U64),
+        ("grid", _P), ("num_slots", _I64), ("channels", _I32), ("grid_size", _I32), ("grid_strides", _I64 * 5),
+        ("in_features", _I32), ("hidden", _I32), ("hidden_layers", _I32), ("out_features", _I32),
+        ("params", _P), ("act", _I32), ("last_act", _I32), ("softplus_beta", _F),
+    ]
+
+
+_PA = ctypes.POINTER(ProjectorArgs)
+SIGNATURES.update({
+    "neat_projector_workspace_bytes": (_SZ, [_PA]),
+    "neat_projector_plan": (ctypes.c_int, [_P, _PA, _P, _SZ]),
+    "neat_projector_fwd": (ctypes.c_int, [_P, _PA, _I32, _P, _P, _SZ]),
+    "neat_projector_bwd": (ctypes.c_int, [_P, _PA, _P, _P, _P, _P, _SZ]),
+})
+
 _LIB = None
 
 
@@ -65,7 +88,7 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built; run `python -m neat_amd.build` (no CPU fallback exists)")
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
             f.restype = res

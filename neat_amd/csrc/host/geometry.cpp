@@ -1,0 +1,237 @@
+#include "geometry.h"
+
+#include <torch/csrc/autograd/custom_function.h>
+
+using namespace neat_host;
+using torch::autograd::AutogradContext;
+using torch::autograd::tensor_list;
+
+namespace
+{
+
+struct IntegrateFn : public torch::autograd::Function<IntegrateFn>
+{
+    static torch::Tensor forward(AutogradContext* ctx, torch::Tensor values, torch::Tensor weight,
+                                 torch::Tensor ray_index, int64_t num_rays, torch::Tensor offsets)
+    {
+        auto v  = values.to(torch::kFloat32).contiguous();
+        auto w  = weight.to(torch::kFloat32).reshape({-1}).contiguous();
+        auto ri = ray_index.to(torch::kLong).reshape({-1}).contiguous();
+        const int64_t S = ri.numel();
+        const int C     = (int)v.size(1);
+        auto out        = torch::empty({C, num_rays}, v.options());
+        NEAT_CALL(neat_integrate_fwd(current_stream(), dptr<float>(v), dptr<float>(w), dptr<int64_t>(ri), S, C,
+                                     num_rays, offsets.defined() ? dptr<int64_t>(offsets) : nullptr,
+                                     dptr<float>(out)));
+        ctx->save_for_backward({v, w, ri});
+        ctx->saved_data["R"] = num_rays;
+        return out;
+    }
+
+    static tensor_list backward(AutogradContext* ctx, tensor_list grads)
+    {
+        auto saved = ctx->get_saved_variables();
+        auto v = saved[0], w = saved[1], ri = saved[2];
+        const int64_t R = ctx->saved_data["R"].toInt();
+        auto g          = grads[0].contiguous();
+        auto gv         = torch::empty_like(v);
+        NEAT_CALL(neat_integrate_bwd(current_stream(), dptr<float>(g), dptr<float>(w), dptr<int64_t>(ri), ri.numel(),
+                                     (int)v.size(1), R, dptr<float>(gv)));
+        torch::Tensor gw;
+        if (ctx->needs_input_grad(1))
+        {
+            // d/dw: sum_c grad[c, ray] * v[i, c]
+            gw = (torch::index_select(g, 1, ri).t() * v).sum(1, true);
+        }
+        return {gv, gw, torch::Tensor(), torch::Tensor(), torch::Tensor()};
+    }
+};
+
+// The fused projector over a canonical segment list: differentiable in the grid and the packed decoder parameters.
+struct ProjectorFn : public torch::autograd::Function<ProjectorFn>
+{
+    static neat_projector_args args(const std::vector<torch::Tensor>& t, const std::vector<int64_t>& iv, double beta)
+    {
+        // t: grid, params, pmin, pmax, prefix, origin, direction, ray_seg_offset, seg_node, seg_ray, seg_count,
+        //    seg_tmin, seg_tmax
+        // iv: num_samples, jitter, seed, in, hidden, layers, out, act, last_act
+        neat_projector_args a{};
+        a.node_pos_min           = dptr<float>(t[2]);
+        a.node_pos_max           = dptr<float>(t[3]);
+        a.node_active_prefix_sum = dptr<int64_t>(t[4]);
+        a.num_nodes              = (int32_t)t[2].size(0);
+        a.ray_origin             = dptr<float>(t[5]);
+        a.ray_direction          = dptr<float>(t[6]);
+        a.num_rays               = t[5].size(0);
+        a.ray_seg_offset         = dptr<int64_t>(t[7]);
+        a.num_segments           = t[8].size(0);
+        a.seg_node               = dptr<int32_t>(t[8]);
+        a.seg_ray                = dptr<int32_t>(t[9]);
+        a.seg_count              = dptr<int32_t>(t[10]);
+        a.seg_tmin               = dptr<float>(t[11]);
+        a.seg_tmax               = dptr<float>(t[12]);
+        a.num_samples            = iv[0];
+        a.jitter                 = (int32_t)iv[1];
+        a.seed                   = (uint64_t)iv[2];
+        a.grid                   = dptr<float>(t[0]);
+        a.num_slots              = t[0].size(0);
+        a.channels               = (int32_t)t[0].size(1);
+        a.grid_size              = (int32_t)t[0].size(2);
+        for (int i = 0; i < 5; ++i) a.grid_strides[i] = t[0].stride(i);
+        a.in_features   = (int32_t)iv[3];
+        a.hidden        = (int32_t)iv[4];
+        a.hidden_layers = (int32_t)iv[5];
+        a.out_features  = (int32_t)iv[6];
+        a.params        = dptr<float>(t[1]);
+        a.act           = (int32_t)iv[7];
+        a.last_act      = (int32_t)iv[8];
+        a.softplus_beta = (float)beta;
+        return a;
+    }
+
+    static torch::Tensor forward(AutogradContext* ctx, torch::Tensor grid, torch::Tensor params,
+                                 std::vector<torch::Tensor> geo, std::vector<int64_t> iv, double beta, bool compensated)
+    {
+        std::vector<torch::Tensor> t = {grid, params.contiguous()};
+        for (auto& g : geo) t.push_back(g);
+        neat_projector_args a = args(t, iv, beta);
+        const size_t bytes    = neat_projector_workspace_bytes(&a);
+        TORCH_CHECK(bytes > 0, "fused projector: ", neat_last_error());
+        auto ws   = torch::empty({(int64_t)bytes}, grid.options().dtype(torch::kUInt8));
+        auto s    = current_stream();
+        auto proj = torch::empty({1, a.num_rays}, grid.options().memory_format(at::MemoryFormat::Contiguous));
+        NEAT_CALL(neat_projector_plan(s, &a, ws.data_ptr(), bytes));
+        NEAT_CALL(neat_projector_fwd(s, &a, compensated ? 1 : 0, dptr<float>(proj), ws.data_ptr(), bytes));
+        t.push_back(ws);
+        ctx->save_for_backward(t);
+        ctx->saved_data["iv"]   = iv;
+        ctx->saved_data["beta"] = beta;
+        return proj;
+    }
+
+    static tensor_list backward(AutogradContext* ctx, tensor_list grads)
+    {
+        auto t            = ctx->get_saved_variables();
+        auto ws           = t.back();
+        auto iv           = ctx->saved_data["iv"].toIntVector();
+        double beta       = ctx->saved_data["beta"].toDouble();
+        neat_projector_args a = args(t, iv, beta);
+        auto gproj        = grads[0].reshape({-1}).contiguous();
+        auto gg           = torch::empty_like(t[0]);
+        auto gp           = torch::empty_like(t[1]);
+        NEAT_CALL(neat_projector_bwd(current_stream(), &a, dptr<float>(gproj), dptr<float>(gg), dptr<float>(gp),
+                                     ws.data_ptr(), (size_t)ws.numel()));
+        // one slot per forward argument (each tensor of `geo` counts); trailing slots stay undefined
+        tensor_list out(2 + 11 + 3);
+        out[0] = gg;
+        out[1] = gp;
+        return out;
+    }
+};
+
+}  // namespace
+
+torch::Tensor IntegrateSamples(torch::Tensor sample_values, torch::Tensor integration_weight, torch::Tensor ray_index,
+                               int num_channels, int num_rays, torch::Tensor ray_sample_offset)
+{
+    if (sample_values.numel() == 0)
+        return torch::zeros({num_channels, num_rays}, torch::TensorOptions(ray_index.device()));
+    TORCH_CHECK(sample_values.dim() == integration_weight.dim());
+    auto v = sample_values.reshape({-1, num_channels});
+    return IntegrateFn::apply(v, integration_weight.reshape({-1}), ray_index.reshape({-1}), num_rays,
+                              ray_sample_offset);
+}
+
+torch::Tensor NeuralGeometry::IntegrateSamplesXRay(torch::Tensor sample_values, torch::Tensor integration_weight,
+                                                   torch::Tensor ray_index, int num_channels, int num_rays)
+{
+    // arbitrary sample order: float atomics like the reference's index_add_
+    return IntegrateSamples(sample_values, integration_weight, ray_index, num_channels, num_rays, {});
+}
+
+HierarchicalNeuralGeometry::HierarchicalNeuralGeometry(int num_channels, int D, std::shared_ptr<CombinedParams> params,
+                                                       HyperTreeBase tree)
+    : NeuralGeometry(num_channels, D, params), tree(tree)
+{
+    TORCH_CHECK(!params->net_params.decoder_skip, "decoder_skip is not supported by this build");
+    decoder = FCBlock(params->net_params.grid_features, num_channels, params->net_params.decoder_hidden_layers,
+                      params->net_params.decoder_hidden_features, true, params->net_params.decoder_activation);
+    register_module("decoder", decoder);
+}
+
+torch::Tensor HierarchicalNeuralGeometry::DecodeFeatures(torch::Tensor neural_features)
+{
+    return decoder->forward_with_last(neural_features, params->net_params.last_activation_function,
+                                      params->net_params.softplus_beta);
+}
+
+torch::Tensor HierarchicalNeuralGeometry::ComputeImage(SampleList all_samples, int num_channels, int num_pixels)
+{
+    if (params->train_params.use_fused_projector && all_samples.segments.defined() && num_channels == 1)
+    {
+        auto img = ComputeImageFused(all_samples, num_pixels);
+        if (img.defined()) return img;
+    }
+    TORCH_CHECK(all_samples.global_coordinate.defined(),
+                "ComputeImage: the SampleList holds only segments and the fused projector does not cover this "
+                "configuration; create it with CreateSamplesForRays");
+    auto sample_output = SampleVolumeIndirect(all_samples.global_coordinate, all_samples.node_id);
+    TORCH_CHECK(sample_output.defined(), "SampleVolumeIndirect is not implemented by this geometry");
+    auto weight = all_samples.weight.unsqueeze(1);
+    // the sampler's output is ray-sorted: segmented, deterministic sums
+    torch::Tensor offsets =
+        all_samples.segments.defined() ? all_samples.segments.ray_sample_offset : torch::Tensor();
+    auto img = IntegrateSamples(sample_output, weight, all_samples.ray_index, num_channels, num_pixels, offsets);
+    TORCH_CHECK(img.size(0) == num_channels);
+    return img;
+}
+
+GeometryExEx::GeometryExEx(int num_channels, int D, HyperTreeBase tree, std::shared_ptr<CombinedParams> params)
+    : HierarchicalNeuralGeometry(num_channels, D, params, tree)
+{
+    std::vector<long> shape(D, params->net_params.grid_size);
+    explicit_grid_generator =
+        ExplicitFeatureGrid(params->octree_params.tree_optimizer_params.max_active_nodes,
+                            params->net_params.grid_features, shape, params->train_params.grid_init);
+    register_module("explicit_grid_generator", explicit_grid_generator);
+}
+
+torch::Tensor GeometryExEx::SampleVolumeIndirect(torch::Tensor global_coordinate, torch::Tensor node_id)
+{
+    auto local_samples   = tree->ComputeLocalSamples(global_coordinate, node_id);
+    auto local_node_id   = tree->GlobalNodeIdToLocalActiveId(node_id);
+    auto neural_features = IndirectGridSample3D(explicit_grid_generator->grid_data, local_node_id, local_samples);
+    return DecodeFeatures(neural_features);
+}
+
+std::string GeometryExEx::FusedUnsupportedReason()
+{
+    auto& np  = params->net_params;
+    auto& g   = explicit_grid_generator->grid_data;
+    const int C = (int)g.size(1), G = (int)g.size(2);
+    if (!(g.size(3) == G && g.size(4) == G)) return "grid is not cubic";
+    if (!g.is_contiguous(at::MemoryFormat::ChannelsLast3d)) return "grid is not channels-last-3d";
+    if (!(C == 4 || C == 8)) return "grid_features not in {4, 8}";
+    if (!(np.decoder_hidden_features == 32 || np.decoder_hidden_features == 64)) return "hidden width not in {32, 64}";
+    if (np.decoder_hidden_layers != 1) return "decoder_hidden_layers != 1";
+    if (!(np.decoder_activation == "relu" || np.decoder_activation == "silu")) return "hidden activation not relu/silu";
+    if (np.last_activation_function != "softplus") return "last activation is not softplus";
+    if (num_channels != 1) return "num_channels != 1";
+    return "";
+}
+
+torch::Tensor GeometryExEx::ComputeImageFused(const SampleList& samples, int num_pixels)
+{
+    if (!FusedUnsupportedReason().empty()) return {};
+    const auto& s = samples.segments;
+    TORCH_CHECK(s.ray_origin.size(0) == num_pixels, "ComputeImage: num_pixels differs from the ray count");
+    auto& np = params->net_params;
+    std::vector<torch::Tensor> geo = {tree->node_position_min, tree->node_position_max, tree->node_active_prefix_sum,
+                                      s.ray_origin, s.ray_direction, s.ray_seg_offset, s.node_id, s.ray, s.count,
+                                      s.tmin, s.tmax};
+    std::vector<int64_t> iv = {s.num_samples, s.jitter ? 1 : 0, (int64_t)s.seed, np.grid_features,
+                               np.decoder_hidden_features, np.decoder_hidden_layers, 1, act_id(np.decoder_activation),
+                               NEAT_ACT_SOFTPLUS};
+    return ProjectorFn::apply(explicit_grid_generator->grid_data, decoder->packed_parameters(), geo, iv,
+                              (double)np.softplus_beta, params->train_params.compensated_sum);
+}

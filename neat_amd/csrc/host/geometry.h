@@ -1,0 +1,71 @@
+// Neural geometry of the projector path (geometry.h:21-249 and geometry_ex_ex.h:64-84 in the reference):
+// NeuralGeometry::IntegrateSamplesXRay, HierarchicalNeuralGeometry::ComputeImage / DecodeFeatures and
+// GeometryExEx::SampleVolumeIndirect, on the C ABI. ComputeImage runs the fused projector when the SampleList carries
+// its canonical segments and the decoder / grid fall in the fused kernels' coverage, the unfused kernel chain
+// otherwise.
+#pragma once
+
+#include <memory>
+
+#include "HyperTree.h"
+#include "ImplicitNet.h"
+#include "IndirectGridSample3D.h"
+#include "Settings.h"
+
+// out[c, ray_index[i]] += sample_values[i, c] * integration_weight[i]; deterministic when ray_sample_offset
+// (CSR offsets of ray-sorted samples) is given.
+torch::Tensor IntegrateSamples(torch::Tensor sample_values, torch::Tensor integration_weight, torch::Tensor ray_index,
+                               int num_channels, int num_rays, torch::Tensor ray_sample_offset = {});
+
+class NeuralGeometry : public torch::nn::Module
+{
+   public:
+    NeuralGeometry(int num_channels, int D, std::shared_ptr<CombinedParams> params)
+        : num_channels(num_channels), D(D), params(params)
+    {
+    }
+
+    // sample_values [num_samples, num_channels] or [groups, group_size, num_channels], integration_weight of the same
+    // rank, ray_index [num_samples] -> [num_channels, num_rays] (geometry.h:112-113, geometry.cpp:393-424).
+    torch::Tensor IntegrateSamplesXRay(torch::Tensor sample_values, torch::Tensor integration_weight,
+                                       torch::Tensor ray_index, int num_channels, int num_rays);
+
+    int num_channels;
+    int D;
+    std::shared_ptr<CombinedParams> params;
+};
+
+class HierarchicalNeuralGeometry : public NeuralGeometry
+{
+   public:
+    HierarchicalNeuralGeometry(int num_channels, int D, std::shared_ptr<CombinedParams> params, HyperTreeBase tree);
+
+    // [num_channels, num_pixels] line integrals of the decoded field over the samples (geometry.cpp:347-389).
+    torch::Tensor ComputeImage(SampleList all_samples, int num_channels, int num_pixels);
+
+    // Input: global_coordinate [num_samples, D], node_id [num_samples] -> density [num_samples, num_channels].
+    virtual torch::Tensor SampleVolumeIndirect(torch::Tensor global_coordinate, torch::Tensor node_id) { return {}; }
+
+    // decoder + last activation (geometry.cpp:102-163), one fused kernel.
+    torch::Tensor DecodeFeatures(torch::Tensor neural_features);
+
+    // The fused projector for a SampleList with segments, or an undefined tensor when not covered.
+    virtual torch::Tensor ComputeImageFused(const SampleList& samples, int num_pixels) { return {}; }
+
+    HyperTreeBase tree = nullptr;
+    FCBlock decoder    = nullptr;
+};
+
+class GeometryExEx : public HierarchicalNeuralGeometry
+{
+   public:
+    GeometryExEx(int num_channels, int D, HyperTreeBase tree, std::shared_ptr<CombinedParams> params);
+
+    torch::Tensor SampleVolumeIndirect(torch::Tensor global_coordinate, torch::Tensor node_id) override;
+    torch::Tensor ComputeImageFused(const SampleList& samples, int num_pixels) override;
+
+    // why the fused projector does not cover this configuration ("" when it does)
+    std::string FusedUnsupportedReason();
+
+    ExplicitFeatureGrid explicit_grid_generator = nullptr;
+};

@@ -9,25 +9,6 @@
 namespace neat
 {
 
-__global__ void __launch_bounds__(256) k_transpose_params(const float* params, MLP m, float* wt)
-{
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= m.nparams()) return;
-    int l = 0;
-    while (l + 1 < m.nlin() && p >= m.wofs(l + 1)) ++l;
-    int rel = p - m.wofs(l);
-    int din = m.din(l), dout = m.dout(l);
-    if (rel < din * dout)
-    {
-        int o = rel / din, k = rel % din;       // params: W[o][k]
-        wt[m.wofs(l) + k * dout + o] = params[p];  // wt: Wt[k][o]
-    }
-    else
-    {
-        wt[p] = params[p];  // bias
-    }
-}
-
 __device__ __forceinline__ void load_x_col(const float* x, int64_t i0, int ns, int in, float* X)
 {
     const int s = threadIdx.x & (kTile - 1);
@@ -82,16 +63,6 @@ __global__ void __launch_bounds__(kTile) k_dec_bwd(const float* x, int64_t S, ML
             for (int k = 0; k < m.in; ++k) gx[(i0 + s) * m.in + k] = t.X[k * kS + s];
         }
     }
-}
-
-// Fixed-order sum of the workgroup partials.
-__global__ void __launch_bounds__(256) k_dec_reduce(const float* partials, int num_wg, int P, float* out)
-{
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    float acc = 0.f;
-    for (int w = 0; w < num_wg; ++w) acc += partials[(int64_t)w * P + p];
-    out[p] = acc;
 }
 
 constexpr int kMaxBwdWG = 1024;
@@ -188,7 +159,7 @@ extern "C"
         else
             k_dec_bwd<64><<<nwg, kTile, 0, s>>>(x, S, m, wt, act, last_act, beta, gy, gx, partials);
         NEAT_LAUNCHED();
-        k_dec_reduce<<<grid_for(P, 256), 256, 0, s>>>(partials, nwg, P, gparams);
+        k_dec_reduce_partials<<<grid_for(P, 256), 256, 0, s>>>(partials, nwg, P, gparams);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }

@@ -61,9 +61,7 @@ __device__ __forceinline__ void dense_col(const float* __restrict__ wt, const fl
     }
     for (int o0 = 0; o0 < dout; o0 += 4)
     {
-        float acc[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] = (o0 + j < dout) ? b[o0 + j] : 0.f;
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int k = 0; k < DIN; ++k)
         {
@@ -74,9 +72,10 @@ __device__ __forceinline__ void dense_col(const float* __restrict__ wt, const fl
                     if (o0 + j < dout) acc[j] = fmaf(wt[k * ks + (o0 + j) * os], h[k], acc[j]);
             }
         }
+        // the bias is added after the sum, like addmm with beta = 1
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            if (o0 + j < dout) out_col[(o0 + j) * kS + s] = acc[j];
+            if (o0 + j < dout) out_col[(o0 + j) * kS + s] = acc[j] + b[o0 + j];
     }
 }
 
@@ -234,6 +233,36 @@ __device__ __forceinline__ void tile_backward(const MLP& m, const float* __restr
     __syncthreads();
     back_col<H>(wt + m.wofs(0), m.in, H, wl, t.A, t.X, false, act, beta);  // X := dL/dx
     __syncthreads();
+}
+
+// Transposed copy of the packed parameters: Wt_l [din, dout] at W_l's offset, biases copied.
+static __global__ void __launch_bounds__(256) k_transpose_params(const float* params, MLP m, float* wt)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m.nparams()) return;
+    int l = 0;
+    while (l + 1 < m.nlin() && p >= m.wofs(l + 1)) ++l;
+    int rel = p - m.wofs(l);
+    int din = m.din(l), dout = m.dout(l);
+    if (rel < din * dout)
+    {
+        int o = rel / din, k = rel % din;       // params: W[o][k]
+        wt[m.wofs(l) + k * dout + o] = params[p];  // wt: Wt[k][o]
+    }
+    else
+    {
+        wt[p] = params[p];  // bias
+    }
+}
+
+// Fixed-order sum of the workgroup partials.
+static __global__ void __launch_bounds__(256) k_dec_reduce_partials(const float* partials, int num_wg, int P, float* out)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float acc = 0.f;
+    for (int w = 0; w < num_wg; ++w) acc += partials[(int64_t)w * P + p];
+    out[p] = acc;
 }
 
 }  // namespace neat

@@ -1,0 +1,830 @@
+// Fused FP32 projector: forward line integrals and backward to grid and decoder gradients over the canonical
+// segment list, with the samples recomputed from the segments (never materialised as a SampleList).
+//
+// It fuses, per sample, the chain the reference runs as separate kernels and ATen ops -- sample position
+// (HyperTree.cu:339-358, :426-432), local coordinates (:177-192), slot lookup (HyperTree.h:183-187), trilinear
+// gather (IndirectGridSample3D.cu:432-572), decoder + softplus (ImplicitNet.h:18-71, geometry.cpp:102-163) and the
+// weighted per-ray sum (geometry.cpp:393-424) -- and the hand-written backward of all of it.
+//
+// Work is node-major: a plan sorts the segments by grid slot (stable, so the order inside a slot is canonical) and
+// cuts every slot's samples into chunks; a workgroup processes one chunk, so all its gathers hit one node's grid.
+//   forward  : one thread per sample, register-resident MLP; per-segment partial sums (sequential in j), then
+//              per-ray sums over the ray's segments in canonical order. Deterministic.
+//   backward : pass 1 recomputes each sample, runs the decoder backward tile by tile (one wave, 64 samples,
+//              LDS-staged weight-gradient reduction, per-workgroup partials) and writes dL/dfeature per sample;
+//              pass 2 scatters dL/dfeature x corner weight into the node's gradient grid held in LDS and stores the
+//              chunk's grid with plain stores; slots cut into several chunks are summed in chunk order. No global
+//              atomics anywhere.
+#include <hipcub/hipcub.hpp>
+
+#include "mlp_tile.hpp"
+
+namespace neat
+{
+
+constexpr int kFwdChunk   = 4096;  // samples per forward / backward-pass-1 chunk
+constexpr int kFwdThreads = 256;
+
+struct ProjGeom
+{
+    const float* pmin;
+    const float* pmax;
+    const int64_t* prefix;
+    int32_t num_nodes;
+    const float* ro;
+    const float* rd;
+    int64_t R;
+    const int64_t* ray_seg_offset;
+    int64_t K;
+    const int32_t* seg_node;
+    const int32_t* seg_ray;
+    const int32_t* seg_count;
+    const float* seg_tmin;
+    const float* seg_tmax;
+    int32_t jitter;
+    uint64_t seed;
+    const float* grid;  // channels-last [N, G, G, G, C]
+    int64_t N;
+    int32_t G;
+    float beta;
+};
+
+struct Plan
+{
+    int32_t* perm;         // [K] node-major position -> canonical segment index
+    int64_t* nm_soff;      // [K + 1] node-major sample offsets
+    int32_t* slot_begin;   // [N + 1] node-major segment range of each slot
+    int64_t* chunk_off1;   // [N + 1] first forward chunk of each slot
+    int64_t* chunk_off2;   // [N + 1] first scatter chunk of each slot
+    int32_t* chunk1_slot;  // [max1]
+    int32_t* chunk1_q0;
+    int32_t* chunk1_q1;
+    int32_t* chunk2_slot;  // [max2]
+    int32_t* chunk2_q0;
+    int32_t* chunk2_q1;
+    int64_t max1, max2;
+    int32_t chunk2;  // samples per scatter chunk
+};
+
+// ---- per-sample geometry (bit-identical to the unfused sampler + ComputeLocalSamples chain) -----------------------
+struct Sample
+{
+    float w;
+    float uv[3];
+    int64_t ray;
+};
+
+__device__ __forceinline__ Sample sample_geo(const ProjGeom& g, int32_t k, int32_t j)
+{
+    Sample s;
+    const int64_t ray  = g.seg_ray[k];
+    const int32_t n    = g.seg_count[k];
+    const int32_t rank = (int32_t)(k - g.ray_seg_offset[ray]);
+    const int32_t node = g.seg_node[k];
+    const float tmin = g.seg_tmin[k], tmax = g.seg_tmax[k];
+    const float step = (tmax - tmin) / (float)n;
+    const float t    = sample_t(tmin, step, j, seg_a(g.jitter, g.seed, ray, rank, j));
+    s.w              = sample_w(tmin, tmax, step, n, j, t, g.jitter, g.seed, ray, rank);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        float p   = g.ro[3 * ray + d] + g.rd[3 * ray + d] * t;
+        float mn  = g.pmin[3 * node + d], mx = g.pmax[3 * node + d];
+        s.uv[d]   = (p - mn) / (mx - mn) * 2 - 1;
+    }
+    s.ray = ray;
+    return s;
+}
+
+// Trilinear corners in a cubic channels-last node grid (offsets in floats from the node's first voxel).
+struct Corners
+{
+    float w[8];
+    int32_t off[8];
+};
+
+__device__ __forceinline__ int clamp_g(int v, int G) { return min(G - 1, max(v, 0)); }
+
+__device__ __forceinline__ Corners corners_cl(const float uv[3], int G, int C)
+{
+    Corners c;
+    float ix = ((uv[0] + 1) / 2) * (float)(G - 1);
+    float iy = ((uv[1] + 1) / 2) * (float)(G - 1);
+    float iz = ((uv[2] + 1) / 2) * (float)(G - 1);
+    int x0 = (int)floorf(ix), y0 = (int)floorf(iy), z0 = (int)floorf(iz);
+    int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+    c.w[0] = ((float)x1 - ix) * ((float)y1 - iy) * ((float)z1 - iz);
+    c.w[1] = (ix - (float)x0) * ((float)y1 - iy) * ((float)z1 - iz);
+    c.w[2] = ((float)x1 - ix) * (iy - (float)y0) * ((float)z1 - iz);
+    c.w[3] = (ix - (float)x0) * (iy - (float)y0) * ((float)z1 - iz);
+    c.w[4] = ((float)x1 - ix) * ((float)y1 - iy) * (iz - (float)z0);
+    c.w[5] = (ix - (float)x0) * ((float)y1 - iy) * (iz - (float)z0);
+    c.w[6] = ((float)x1 - ix) * (iy - (float)y0) * (iz - (float)z0);
+    c.w[7] = (ix - (float)x0) * (iy - (float)y0) * (iz - (float)z0);
+    const int cx0 = clamp_g(x0, G), cx1 = clamp_g(x1, G);
+    const int cy0 = clamp_g(y0, G), cy1 = clamp_g(y1, G);
+    const int cz0 = clamp_g(z0, G), cz1 = clamp_g(z1, G);
+    const int zs = G * G * C, ys = G * C;
+    c.off[0] = cz0 * zs + cy0 * ys + cx0 * C;
+    c.off[1] = cz0 * zs + cy0 * ys + cx1 * C;
+    c.off[2] = cz0 * zs + cy1 * ys + cx0 * C;
+    c.off[3] = cz0 * zs + cy1 * ys + cx1 * C;
+    c.off[4] = cz1 * zs + cy0 * ys + cx0 * C;
+    c.off[5] = cz1 * zs + cy0 * ys + cx1 * C;
+    c.off[6] = cz1 * zs + cy1 * ys + cx0 * C;
+    c.off[7] = cz1 * zs + cy1 * ys + cx1 * C;
+    return c;
+}
+
+template <int C>
+__device__ __forceinline__ void gather_cl(const float* node_grid, const Corners& c, float (&x)[C])
+{
+#pragma unroll
+    for (int q = 0; q < C; q += 4)
+    {
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            float4 v = *reinterpret_cast<const float4*>(node_grid + c.off[k] + q);
+            acc.x += v.x * c.w[k];
+            acc.y += v.y * c.w[k];
+            acc.z += v.z * c.w[k];
+            acc.w += v.w * c.w[k];
+        }
+        x[q]     = acc.x;
+        x[q + 1] = acc.y;
+        x[q + 2] = acc.z;
+        x[q + 3] = acc.w;
+    }
+}
+
+// Register-resident decoder C -> H -> H -> 1 (one hidden layer), packed parameters read with uniform loads.
+// Returns the output pre-activation.
+template <int C, int H, int ACT>
+__device__ __forceinline__ float mlp_forward_reg(const float* __restrict__ P, const float (&x)[C], float beta)
+{
+    const float* W0 = P;
+    const float* b0 = W0 + H * C;
+    const float* W1 = b0 + H;
+    const float* b1 = W1 + H * H;
+    const float* W2 = b1 + H;
+    const float* b2 = W2 + H;
+    float h0[H];
+#pragma unroll
+    for (int o = 0; o < H; ++o)
+    {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < C; ++k) acc = fmaf(W0[o * C + k], x[k], acc);
+        h0[o] = act_f(ACT, acc + b0[o], beta);
+    }
+    float acc2 = 0.f;
+#pragma unroll
+    for (int o = 0; o < H; ++o)
+    {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < H; ++k) acc = fmaf(W1[o * H + k], h0[k], acc);
+        float h1 = act_f(ACT, acc + b1[o], beta);
+        acc2     = fmaf(W2[o], h1, acc2);
+    }
+    return acc2 + b2[0];
+}
+
+// Node-major position of the segment holding chunk-relative sample s: the last q in [q0, q1) with
+// nm_soff[q] - base <= s.
+__device__ __forceinline__ int32_t find_segment(const int64_t* nm_soff, int32_t q0, int32_t q1, int64_t target)
+{
+    int32_t lo = q0, hi = q1 - 1;
+    while (lo < hi)
+    {
+        int32_t mid = (lo + hi + 1) >> 1;
+        if (nm_soff[mid] <= target)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---- plan ---------------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_plan_keys(ProjGeom g, int32_t* keys, int32_t* vals)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.K) return;
+    int32_t node = g.seg_node[k];
+    int64_t slot = (node >= 0 && node < g.num_nodes) ? g.prefix[node] : -1;
+    keys[k]      = (slot >= 0 && slot < g.N) ? (int32_t)slot : (int32_t)g.N;  // invalid slots sort last
+    vals[k]      = (int32_t)k;
+}
+
+__global__ void __launch_bounds__(256) k_plan_counts(ProjGeom g, const int32_t* perm, int64_t* nm_soff)
+{
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < g.K) nm_soff[q] = g.seg_count[perm[q]];
+}
+
+__global__ void __launch_bounds__(256) k_plan_slots(ProjGeom g, const int32_t* skeys, Plan p)
+{
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > g.N) return;
+    // lower bound of s in the sorted keys
+    int64_t lo = 0, hi = g.K;
+    while (lo < hi)
+    {
+        int64_t mid = (lo + hi) >> 1;
+        if (skeys[mid] < s)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    p.slot_begin[s] = (int32_t)lo;
+}
+
+__global__ void __launch_bounds__(256) k_plan_chunk_counts(ProjGeom g, Plan p)
+{
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.N) return;
+    int64_t n       = p.nm_soff[p.slot_begin[s + 1]] - p.nm_soff[p.slot_begin[s]];
+    p.chunk_off1[s] = (n + kFwdChunk - 1) / kFwdChunk;
+    p.chunk_off2[s] = (n + p.chunk2 - 1) / p.chunk2;
+}
+
+// First node-major segment in [lo, hi) whose first sample is >= target.
+__device__ __forceinline__ int32_t lower_seg(const int64_t* nm_soff, int32_t lo, int32_t hi, int64_t target)
+{
+    while (lo < hi)
+    {
+        int32_t mid = (lo + hi) >> 1;
+        if (nm_soff[mid] < target)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// One thread per chunk: chunk r of a slot covers the slot's segments whose first sample lies in [r*T, (r+1)*T)
+// (relative to the slot's first sample). Threads [0, max1) fill forward chunks, [max1, max1 + max2) scatter chunks.
+__global__ void __launch_bounds__(256) k_plan_chunks(ProjGeom g, Plan p)
+{
+    int64_t i       = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool pass1 = i < p.max1;
+    if (!pass1) i -= p.max1;
+    const int64_t* off = pass1 ? p.chunk_off1 : p.chunk_off2;
+    if (i >= (pass1 ? p.max1 : p.max2) || i >= off[g.N]) return;
+    // slot owning chunk i: last s with off[s] <= i
+    int64_t lo = 0, hi = g.N - 1;
+    while (lo < hi)
+    {
+        int64_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int64_t s    = lo;
+    const int64_t r    = i - off[s];
+    const int64_t T    = pass1 ? kFwdChunk : p.chunk2;
+    const int32_t sb   = p.slot_begin[s], se = p.slot_begin[s + 1];
+    const int64_t base = p.nm_soff[sb];
+    const int32_t q0   = lower_seg(p.nm_soff, sb, se, base + r * T);
+    const int32_t q1   = (i + 1 < off[s + 1]) ? lower_seg(p.nm_soff, sb, se, base + (r + 1) * T) : se;
+    (pass1 ? p.chunk1_slot : p.chunk2_slot)[i] = (int32_t)s;
+    (pass1 ? p.chunk1_q0 : p.chunk2_q0)[i]     = q0;
+    (pass1 ? p.chunk1_q1 : p.chunk2_q1)[i]     = q1;
+}
+
+// ---- forward --------------------------------------------------------------------------------------------------------
+template <int C, int H, int ACT>
+__global__ void __launch_bounds__(kFwdThreads) k_proj_fwd(ProjGeom g, Plan p, const float* __restrict__ params,
+                                                          float* seg_partial)
+{
+    // The chunk's samples are processed in windows of kFwdChunk (a chunk exceeds one window only when it ends with
+    // a segment longer than the remaining space); each segment's sum is accumulated window by window, sequentially
+    // in j inside a window. A chunk holds at most kFwdChunk segments (every segment starts inside it).
+    __shared__ float s_contrib[kFwdChunk];
+    __shared__ float s_seg[kFwdChunk];
+    const int64_t c = blockIdx.x;
+    if (c >= p.chunk_off1[g.N]) return;
+    const int32_t slot = p.chunk1_slot[c];
+    const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c];
+    const int64_t base  = p.nm_soff[q0];
+    const int64_t nsamp = p.nm_soff[q1] - base;
+    const float* node_grid = g.grid + (int64_t)slot * g.G * g.G * g.G * C;
+    for (int32_t q = q0 + threadIdx.x; q < q1; q += kFwdThreads) s_seg[q - q0] = 0.f;
+    for (int64_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
+    {
+        const int32_t wn = (int32_t)min<int64_t>(kFwdChunk, nsamp - w0);
+        for (int32_t i = threadIdx.x; i < wn; i += kFwdThreads)
+        {
+            const int64_t s = w0 + i;
+            const int32_t q = find_segment(p.nm_soff, q0, q1, base + s);
+            const int32_t k = p.perm[q];
+            const int32_t j = (int32_t)(base + s - p.nm_soff[q]);
+            Sample smp      = sample_geo(g, k, j);
+            Corners cr      = corners_cl(smp.uv, g.G, C);
+            float x[C];
+            gather_cl<C>(node_grid, cr, x);
+            float z      = mlp_forward_reg<C, H, ACT>(params, x, g.beta);
+            s_contrib[i] = act_f(NEAT_ACT_SOFTPLUS, z, g.beta) * smp.w;
+        }
+        __syncthreads();
+        for (int32_t q = q0 + threadIdx.x; q < q1; q += kFwdThreads)
+        {
+            const int64_t a = max<int64_t>(p.nm_soff[q] - base, w0);
+            const int64_t b = min<int64_t>(p.nm_soff[q + 1] - base, w0 + wn);
+            if (a >= b) continue;
+            float acc = 0.f;
+            for (int64_t s = a; s < b; ++s) acc += s_contrib[s - w0];
+            s_seg[q - q0] += acc;
+        }
+        __syncthreads();
+    }
+    for (int32_t q = q0 + threadIdx.x; q < q1; q += kFwdThreads) seg_partial[p.perm[q]] = s_seg[q - q0];
+}
+
+// proj[r] = sum of the ray's segment partials in canonical order (optionally Neumaier-compensated).
+__global__ void __launch_bounds__(256) k_proj_raysum(const int64_t* ray_seg_offset, int64_t R,
+                                                     const float* seg_partial, int32_t compensated, float* proj)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    float sum = 0.f, comp = 0.f;
+    for (int64_t k = ray_seg_offset[r]; k < ray_seg_offset[r + 1]; ++k)
+    {
+        float v = seg_partial[k];
+        if (compensated)
+        {
+            float t = sum + v;
+            comp += (fabsf(sum) >= fabsf(v)) ? (sum - t) + v : (v - t) + sum;
+            sum = t;
+        }
+        else
+        {
+            sum += v;
+        }
+    }
+    proj[r] = sum + comp;
+}
+
+// ---- backward pass 1: decoder backward per tile, dL/dfeature per sample ------------------------------------------------
+template <int C, int H>
+__global__ void __launch_bounds__(kTile) k_proj_bw1(ProjGeom g, Plan p, MLP m, const float* __restrict__ wt, int act,
+                                                    const float* grad_proj, float* dfeat, float* partials)
+{
+    __shared__ float lds[kTileFloats];
+    TileBufs t     = tile_bufs(lds);
+    float* gacc    = partials + (int64_t)blockIdx.x * m.nparams();
+    const int lane = threadIdx.x;
+    const int64_t nchunks = p.chunk_off1[g.N];
+    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x)
+    {
+        const int32_t slot = p.chunk1_slot[c];
+        const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c];
+        const int64_t base  = p.nm_soff[q0];
+        const int32_t nsamp = (int32_t)(p.nm_soff[q1] - base);
+        const float* node_grid = g.grid + (int64_t)slot * g.G * g.G * g.G * C;
+        for (int32_t s0 = 0; s0 < nsamp; s0 += kTile)
+        {
+            const int32_t s  = s0 + lane;
+            const bool valid = s < nsamp;
+            const int ns     = min(kTile, nsamp - s0);
+            float gy         = 0.f;
+            float x[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) x[q] = 0.f;
+            if (valid)
+            {
+                const int32_t q = find_segment(p.nm_soff, q0, q1, base + s);
+                const int32_t k = p.perm[q];
+                const int32_t j = (int32_t)(base + s - p.nm_soff[q]);
+                Sample smp      = sample_geo(g, k, j);
+                Corners cr      = corners_cl(smp.uv, g.G, C);
+                gather_cl<C>(node_grid, cr, x);
+                gy = grad_proj[smp.ray] * smp.w;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < kMaxIn; ++q) t.X[q * kS + lane] = q < C ? x[q] : 0.f;
+            __syncthreads();
+            tile_forward<H>(m, wt, WLayout{true}, act, g.beta, t);
+            t.D[lane] = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, t.D[lane], g.beta) : 0.f;
+            tile_backward<H>(m, wt, WLayout{true}, act, g.beta, t, ns, gacc);
+            if (valid)
+            {
+                float* out = dfeat + (base + s) * C;
+#pragma unroll
+                for (int q = 0; q < C; q += 4)
+                    *reinterpret_cast<float4*>(out + q) =
+                        make_float4(t.X[q * kS + lane], t.X[(q + 1) * kS + lane], t.X[(q + 2) * kS + lane],
+                                    t.X[(q + 3) * kS + lane]);
+            }
+        }
+    }
+}
+
+// ---- backward pass 2: scatter into the node's gradient grid in LDS ----------------------------------------------------
+template <int C>
+__global__ void __launch_bounds__(256) k_proj_bw2(ProjGeom g, Plan p, const float* dfeat, float* grad_grid,
+                                                  float* partial_grids)
+{
+    extern __shared__ float sg[];
+    const int64_t c = blockIdx.x;
+    if (c >= p.chunk_off2[g.N]) return;
+    const int32_t slot   = p.chunk2_slot[c];
+    const int32_t q0     = p.chunk2_q0[c], q1 = p.chunk2_q1[c];
+    const int32_t gsize  = g.G * g.G * g.G * C;
+    for (int32_t i = threadIdx.x; i < gsize; i += blockDim.x) sg[i] = 0.f;
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
+    for (int32_t q = q0 + wave; q < q1; q += nwave)
+    {
+        const int32_t k     = p.perm[q];
+        const int32_t n     = g.seg_count[k];
+        const int64_t sbase = p.nm_soff[q];
+        for (int32_t j = lane; j < n; j += 64)
+        {
+            Sample smp       = sample_geo(g, k, j);
+            Corners cr       = corners_cl(smp.uv, g.G, C);
+            const float* df  = dfeat + (sbase + j) * C;
+            float dv[C];
+#pragma unroll
+            for (int q2 = 0; q2 < C; q2 += 4)
+            {
+                float4 v    = *reinterpret_cast<const float4*>(df + q2);
+                dv[q2]      = v.x;
+                dv[q2 + 1]  = v.y;
+                dv[q2 + 2]  = v.z;
+                dv[q2 + 3]  = v.w;
+            }
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk)
+            {
+#pragma unroll
+                for (int q2 = 0; q2 < C; ++q2) atomicAdd(sg + cr.off[kk] + q2, cr.w[kk] * dv[q2]);
+            }
+        }
+    }
+    __syncthreads();
+    const bool split = p.chunk_off2[slot + 1] - p.chunk_off2[slot] > 1;
+    float* dst       = split ? partial_grids + c * gsize : grad_grid + (int64_t)slot * gsize;
+    for (int32_t i = threadIdx.x * 4; i < gsize; i += blockDim.x * 4)
+        *reinterpret_cast<float4*>(dst + i) = make_float4(sg[i], sg[i + 1], sg[i + 2], sg[i + 3]);
+}
+
+// Slots cut into several scatter chunks: sum the chunk grids in chunk order. Slots without samples: zero.
+__global__ void __launch_bounds__(256) k_proj_bw2_reduce(ProjGeom g, Plan p, int32_t gsize, const float* partial_grids,
+                                                         float* grad_grid)
+{
+    const int64_t slot = blockIdx.y;
+    const int32_t i    = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= gsize) return;
+    const int64_t c0 = p.chunk_off2[slot], c1 = p.chunk_off2[slot + 1];
+    if (c1 - c0 == 1) return;
+    float acc = 0.f;
+    for (int64_t c = c0; c < c1; ++c) acc += partial_grids[c * gsize + i];
+    grad_grid[slot * gsize + i] = acc;
+}
+
+__global__ void __launch_bounds__(256) k_fill_f32(float* p, int64_t n, float v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---- workspace layout ---------------------------------------------------------------------------------------------------
+struct Layout
+{
+    size_t off[24];
+    size_t total;
+    int n = 0;
+    size_t add(size_t bytes)
+    {
+        size_t o  = total;
+        off[n++]  = o;
+        total     = o + ((bytes + 255) / 256) * 256;
+        return o;
+    }
+};
+
+struct ProjSizes
+{
+    int64_t K, S, N;
+    int32_t G, C, P;
+    int64_t max1, max2;
+    int32_t chunk2;
+    size_t sort_temp;
+};
+
+int32_t scatter_chunk(int64_t S)
+{
+    // aim for >= ~1024 scatter workgroups, but keep chunks between 2048 and 65536 samples
+    int64_t t = S / 1024;
+    return (int32_t)std::max<int64_t>(2048, std::min<int64_t>(65536, t));
+}
+
+ProjSizes proj_sizes(int64_t K, int64_t S, int64_t N, int32_t G, int32_t C, int32_t P)
+{
+    ProjSizes z{K, S, N, G, C, P, 0, 0, scatter_chunk(S), 0};
+    z.max1 = (S + kFwdChunk - 1) / kFwdChunk + N;
+    z.max2 = (S + z.chunk2 - 1) / z.chunk2 + N;
+    size_t tmp = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
+                                       (int32_t*)nullptr, (int)std::max<int64_t>(K, 1), 0, 32);
+    z.sort_temp = tmp;
+    return z;
+}
+
+// plan arrays, then forward / backward scratch
+Layout proj_layout(const ProjSizes& z)
+{
+    Layout L{};
+    L.add(4 * z.K);                       // 0 perm
+    L.add(8 * (z.K + 1));                 // 1 nm_soff
+    L.add(4 * (z.N + 1));                 // 2 slot_begin
+    L.add(8 * (z.N + 1));                 // 3 chunk_off1
+    L.add(8 * (z.N + 1));                 // 4 chunk_off2
+    L.add(4 * z.max1 * 3);                // 5 chunk1 slot/q0/q1
+    L.add(4 * z.max2 * 3);                // 6 chunk2 slot/q0/q1
+    L.add(4 * z.K);                       // 7 keys (sort in)
+    L.add(4 * z.K);                       // 8 keys (sorted)
+    L.add(4 * z.K);                       // 9 vals (sort in)
+    L.add(std::max<size_t>(z.sort_temp, scan_workspace_bytes(z.K + 1)));  // 10 sort / scan temp
+    L.add(4 * z.K);                       // 11 seg_partial (forward)
+    L.add(4 * (size_t)z.P);               // 12 transposed decoder params
+    L.add(4 * (size_t)z.S * z.C);         // 13 dfeat
+    return L;
+}
+
+int bw1_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 2048)); }
+
+size_t proj_bytes(const ProjSizes& z)
+{
+    Layout L = proj_layout(z);
+    size_t partials = (size_t)bw1_workgroups(z) * z.P * 4;
+    size_t grids    = (size_t)z.max2 * z.G * z.G * z.G * z.C * 4;
+    return L.total + ((std::max(partials, (size_t)0) + 255) / 256) * 256 + grids + 256;
+}
+
+Plan plan_view(void* ws, const ProjSizes& z)
+{
+    Layout L = proj_layout(z);
+    char* b  = static_cast<char*>(ws);
+    Plan p;
+    p.perm        = reinterpret_cast<int32_t*>(b + L.off[0]);
+    p.nm_soff     = reinterpret_cast<int64_t*>(b + L.off[1]);
+    p.slot_begin  = reinterpret_cast<int32_t*>(b + L.off[2]);
+    p.chunk_off1  = reinterpret_cast<int64_t*>(b + L.off[3]);
+    p.chunk_off2  = reinterpret_cast<int64_t*>(b + L.off[4]);
+    int32_t* c1   = reinterpret_cast<int32_t*>(b + L.off[5]);
+    int32_t* c2   = reinterpret_cast<int32_t*>(b + L.off[6]);
+    p.chunk1_slot = c1;
+    p.chunk1_q0   = c1 + z.max1;
+    p.chunk1_q1   = c1 + 2 * z.max1;
+    p.chunk2_slot = c2;
+    p.chunk2_q0   = c2 + z.max2;
+    p.chunk2_q1   = c2 + 2 * z.max2;
+    p.max1        = z.max1;
+    p.max2        = z.max2;
+    p.chunk2      = z.chunk2;
+    return p;
+}
+
+}  // namespace neat
+
+using namespace neat;
+
+namespace
+{
+
+struct Args
+{
+    ProjGeom g;
+    MLP m;
+    int act;
+    const float* params;
+    ProjSizes z;
+};
+
+int make_args(Args& a, const neat_projector_args* x)
+{
+    NEAT_REQUIRE(x, "projector: null argument struct");
+    const int64_t R = x->num_rays, K = x->num_segments, S = x->num_samples, N = x->num_slots;
+    const int32_t C = x->channels, G = x->grid_size;
+    NEAT_REQUIRE(R >= 0 && K >= 0 && S >= 0 && N > 0 && x->num_nodes > 0 && G >= 2 && C > 0,
+                 "projector: bad sizes");
+    NEAT_REQUIRE(x->node_pos_min && x->node_pos_max && x->node_active_prefix_sum && x->grid && x->params,
+                 "projector: null pointer");
+    NEAT_REQUIRE(R == 0 || (x->ray_origin && x->ray_direction && x->ray_seg_offset), "projector: null ray arrays");
+    NEAT_REQUIRE(K == 0 || (x->seg_node && x->seg_ray && x->seg_count && x->seg_tmin && x->seg_tmax),
+                 "projector: null segment arrays");
+    NEAT_REQUIRE(K < INT32_MAX && S < ((int64_t)1 << 40), "projector: too many segments");
+    const int64_t* st = x->grid_strides;
+    const int64_t gs  = (int64_t)G * G * G * C;
+    if (!(st[1] == 1 && st[4] == C && st[3] == (int64_t)G * C && st[2] == (int64_t)G * G * C && st[0] == gs))
+        return fail(NEAT_E_UNSUPPORTED, "projector: the grid must be cubic and channels-last contiguous");
+    if (!(x->in_features == C && (C == 4 || C == 8) && (x->hidden == 32 || x->hidden == 64) &&
+          x->hidden_layers == 1 && x->out_features == 1 && (x->act == NEAT_ACT_RELU || x->act == NEAT_ACT_SILU) &&
+          x->last_act == NEAT_ACT_SOFTPLUS))
+        return fail(NEAT_E_UNSUPPORTED,
+                    "projector: fused kernels cover in=C in {4,8}, hidden {32,64}, one hidden layer, out 1, "
+                    "relu/silu hidden activations and a softplus output (got in=%d C=%d hidden=%d layers=%d out=%d "
+                    "act=%d last=%d)",
+                    x->in_features, C, x->hidden, x->hidden_layers, x->out_features, x->act, x->last_act);
+    NEAT_REQUIRE((reinterpret_cast<uintptr_t>(x->grid) % 16) == 0, "projector: grid must be 16-byte aligned");
+    a.g = ProjGeom{x->node_pos_min, x->node_pos_max, x->node_active_prefix_sum, x->num_nodes, x->ray_origin,
+                   x->ray_direction, R, x->ray_seg_offset, K, x->seg_node, x->seg_ray, x->seg_count, x->seg_tmin,
+                   x->seg_tmax, x->jitter, x->seed, x->grid, N, G, x->softplus_beta};
+    a.m      = MLP{x->in_features, x->hidden, x->hidden_layers, x->out_features};
+    a.act    = x->act;
+    a.params = x->params;
+    a.z      = proj_sizes(K, S, N, G, C, a.m.nparams());
+    return NEAT_OK;
+}
+
+int check_ws(const Args& a, void* ws, size_t bytes)
+{
+    size_t need = proj_bytes(a.z);
+    if (!ws || bytes < need) return fail(NEAT_E_OVERFLOW, "projector workspace too small: %zu < %zu", bytes, need);
+    if (reinterpret_cast<uintptr_t>(ws) % 256) return fail(NEAT_E_ARG, "projector workspace must be 256-byte aligned");
+    return NEAT_OK;
+}
+
+char* region(void* ws, const ProjSizes& z, int i) { return static_cast<char*>(ws) + proj_layout(z).off[i]; }
+
+template <int C, int H>
+int launch_fwd_ch(hipStream_t s, const Args& a, const Plan& p, float* seg_partial)
+{
+    unsigned grid = (unsigned)std::max<int64_t>(1, a.z.max1);
+    if (a.act == NEAT_ACT_RELU)
+        k_proj_fwd<C, H, NEAT_ACT_RELU><<<grid, kFwdThreads, 0, s>>>(a.g, p, a.params, seg_partial);
+    else
+        k_proj_fwd<C, H, NEAT_ACT_SILU><<<grid, kFwdThreads, 0, s>>>(a.g, p, a.params, seg_partial);
+    NEAT_LAUNCHED();
+    return NEAT_OK;
+}
+
+template <int C, int H>
+int launch_bw_ch(hipStream_t s, const Args& a, const Plan& p, const float* grad_proj, float* wt, float* dfeat,
+                 float* partials, float* grid_partials, float* grad_grid, float* grad_params)
+{
+    const int nwg = bw1_workgroups(a.z);
+    const int P   = a.m.nparams();
+    NEAT_HIP(hipMemsetAsync(partials, 0, (size_t)nwg * P * sizeof(float), s));
+    k_proj_bw1<C, H><<<nwg, kTile, 0, s>>>(a.g, p, a.m, wt, a.act, grad_proj, dfeat, partials);
+    NEAT_LAUNCHED();
+    k_dec_reduce_partials<<<grid_for(P, 256), 256, 0, s>>>(partials, nwg, P, grad_params);
+    NEAT_LAUNCHED();
+    const int32_t gsize = a.g.G * a.g.G * a.g.G * C;
+    const size_t smem   = (size_t)gsize * sizeof(float);
+    NEAT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_proj_bw2<C>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    k_proj_bw2<C><<<(unsigned)std::max<int64_t>(1, a.z.max2), 256, smem, s>>>(a.g, p, dfeat, grad_grid,
+                                                                               grid_partials);
+    NEAT_LAUNCHED();
+    dim3 rg(grid_for(gsize, 256), (unsigned)a.g.N);
+    k_proj_bw2_reduce<<<rg, 256, 0, s>>>(a.g, p, gsize, grid_partials, grad_grid);
+    NEAT_LAUNCHED();
+    return NEAT_OK;
+}
+
+#define NEAT_PROJ_DISPATCH(CALL)                                   \
+    do                                                             \
+    {                                                              \
+        const int C_ = a.g.N ? (int)a.m.in : 0, H_ = a.m.hidden;   \
+        if (C_ == 8 && H_ == 64) return CALL(8, 64);               \
+        if (C_ == 8 && H_ == 32) return CALL(8, 32);               \
+        if (C_ == 4 && H_ == 64) return CALL(4, 64);               \
+        if (C_ == 4 && H_ == 32) return CALL(4, 32);               \
+    } while (0)
+
+}  // namespace
+
+extern "C"
+{
+    size_t neat_projector_workspace_bytes(const neat_projector_args* x)
+    {
+        Args a;
+        if (make_args(a, x) != NEAT_OK) return 0;
+        return proj_bytes(a.z);
+    }
+
+    int neat_projector_plan(neat_stream_t stream, const neat_projector_args* x, void* ws, size_t bytes)
+    {
+        Args a;
+        int st = make_args(a, x);
+        if (st) return st;
+        if ((st = check_ws(a, ws, bytes))) return st;
+        hipStream_t s   = as_stream(stream);
+        const auto& z   = a.z;
+        Plan p          = plan_view(ws, z);
+        int32_t* keys   = reinterpret_cast<int32_t*>(region(ws, z, 7));
+        int32_t* skeys  = reinterpret_cast<int32_t*>(region(ws, z, 8));
+        int32_t* vals   = reinterpret_cast<int32_t*>(region(ws, z, 9));
+        void* temp      = region(ws, z, 10);
+        size_t temp_sz  = proj_layout(z).off[11] - proj_layout(z).off[10];
+        if (z.K > 0)
+        {
+            k_plan_keys<<<grid_for(z.K, 256), 256, 0, s>>>(a.g, keys, vals);
+            NEAT_LAUNCHED();
+            int end_bit = 1;
+            while (((int64_t)1 << end_bit) <= z.N) ++end_bit;
+            size_t tmp = z.sort_temp;
+            NEAT_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tmp, keys, skeys, vals, p.perm, (int)z.K, 0, end_bit,
+                                                        s));
+            k_plan_counts<<<grid_for(z.K, 256), 256, 0, s>>>(a.g, p.perm, p.nm_soff);
+            NEAT_LAUNCHED();
+        }
+        if ((st = exclusive_scan_i64(s, p.nm_soff, z.K, temp, temp_sz))) return st;
+        if (z.K > 0)
+        {
+            k_plan_slots<<<grid_for(z.N + 1, 256), 256, 0, s>>>(a.g, skeys, p);
+            NEAT_LAUNCHED();
+        }
+        else
+        {
+            NEAT_HIP(hipMemsetAsync(p.slot_begin, 0, (size_t)(z.N + 1) * sizeof(int32_t), s));
+        }
+        k_plan_chunk_counts<<<grid_for(z.N, 256), 256, 0, s>>>(a.g, p);
+        NEAT_LAUNCHED();
+        if ((st = exclusive_scan_i64(s, p.chunk_off1, z.N, temp, temp_sz))) return st;
+        if ((st = exclusive_scan_i64(s, p.chunk_off2, z.N, temp, temp_sz))) return st;
+        k_plan_chunks<<<grid_for(z.max1 + z.max2, 256), 256, 0, s>>>(a.g, p);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
+
+    int neat_projector_fwd(neat_stream_t stream, const neat_projector_args* x, int32_t compensated, float* proj,
+                           void* ws, size_t bytes)
+    {
+        Args a;
+        int st = make_args(a, x);
+        if (st) return st;
+        if ((st = check_ws(a, ws, bytes))) return st;
+        NEAT_REQUIRE(proj || a.z.K == 0 && x->num_rays == 0, "projector: null output");
+        hipStream_t s      = as_stream(stream);
+        Plan p             = plan_view(ws, a.z);
+        float* seg_partial = reinterpret_cast<float*>(region(ws, a.z, 11));
+        if (a.z.K > 0)
+        {
+            // segments whose node has no valid slot are in no chunk: they contribute zero
+            k_fill_f32<<<grid_for(a.z.K, 256), 256, 0, s>>>(seg_partial, a.z.K, 0.f);
+            NEAT_LAUNCHED();
+#define FWD_CALL(C, H) launch_fwd_ch<C, H>(s, a, p, seg_partial)
+            st = NEAT_OK;
+            [&]() -> int {
+                NEAT_PROJ_DISPATCH(st = FWD_CALL);
+                return 0;
+            }();
+#undef FWD_CALL
+            if (st) return st;
+        }
+        if (x->num_rays > 0)
+        {
+            k_proj_raysum<<<grid_for(x->num_rays, 256), 256, 0, s>>>(x->ray_seg_offset, x->num_rays, seg_partial,
+                                                                      compensated, proj);
+            NEAT_LAUNCHED();
+        }
+        return NEAT_OK;
+    }
+
+    int neat_projector_bwd(neat_stream_t stream, const neat_projector_args* x, const float* grad_proj,
+                           float* grad_grid, float* grad_params, void* ws, size_t bytes)
+    {
+        Args a;
+        int st = make_args(a, x);
+        if (st) return st;
+        if ((st = check_ws(a, ws, bytes))) return st;
+        NEAT_REQUIRE(grad_grid && grad_params && (grad_proj || x->num_rays == 0), "projector: null gradient buffer");
+        hipStream_t s       = as_stream(stream);
+        const auto& z       = a.z;
+        Plan p              = plan_view(ws, z);
+        float* wt           = reinterpret_cast<float*>(region(ws, z, 12));
+        float* dfeat        = reinterpret_cast<float*>(region(ws, z, 13));
+        const Layout L      = proj_layout(z);
+        float* partials     = reinterpret_cast<float*>(static_cast<char*>(ws) + L.total);
+        const size_t pbytes = ((size_t)bw1_workgroups(z) * z.P * 4 + 255) / 256 * 256;
+        float* gpartials    = reinterpret_cast<float*>(static_cast<char*>(ws) + L.total + pbytes);
+        k_transpose_params<<<grid_for(z.P, 256), 256, 0, s>>>(a.params, a.m, wt);
+        NEAT_LAUNCHED();
+        if (z.K == 0)
+        {
+            k_fill_f32<<<grid_for(z.N * z.G * z.G * z.G * z.C, 256), 256, 0, s>>>(grad_grid,
+                                                                               z.N * z.G * z.G * z.G * z.C, 0.f);
+            NEAT_LAUNCHED();
+            k_fill_f32<<<grid_for(z.P, 256), 256, 0, s>>>(grad_params, z.P, 0.f);
+            NEAT_LAUNCHED();
+            return NEAT_OK;
+        }
+#define BW_CALL(C, H) launch_bw_ch<C, H>(s, a, p, grad_proj, wt, dfeat, partials, gpartials, grad_grid, grad_params)
+        st = NEAT_OK;
+        [&]() -> int {
+            NEAT_PROJ_DISPATCH(st = BW_CALL);
+            return 0;
+        }();
+#undef BW_CALL
+        return st;
+    }
+}

@@ -2,6 +2,8 @@
 
 These are what the parity tests drive directly; the libtorch module layer (_neat_host) calls the same entry points.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -155,3 +157,56 @@ def mse_fp64(pred, target, grad_scale=1.0, want_grad=True, stream=None):
     call("neat_mse_fp64", stream_ptr(stream), ptr(pred.contiguous()), ptr(target.contiguous()), n, grad_scale,
          ptr(loss), ptr(grad), ptr(ws), wsb)
     return loss, grad
+
+
+class Projector:
+    """Fused projector over one batch's canonical segment list (neat_projector_plan / _fwd / _bwd).
+
+    Holds references to every tensor whose address is in the argument struct, and the workspace the plan fills."""
+
+    def __init__(self, pos_min, pos_max, prefix, origin, direction, segs, grid, params, mlp, jitter=False, seed=0,
+                 stream=None):
+        self.stream = stream
+        N, C, G = grid.shape[0], grid.shape[1], grid.shape[2]
+        R = origin.shape[0]
+        K = segs["seg_node"].shape[0]
+        S = int(segs["ray_sample_offset"][R].item()) if "num_samples" not in segs else segs["num_samples"]
+        self._keep = [pos_min, pos_max, prefix, origin, direction, grid, params] + [segs[k] for k in (
+            "ray_seg_offset", "seg_node", "seg_ray", "seg_count", "seg_tmin", "seg_tmax")]
+        a = _abi.ProjectorArgs()
+        a.node_pos_min, a.node_pos_max, a.node_active_prefix_sum = pos_min.data_ptr(), pos_max.data_ptr(), \
+            prefix.data_ptr()
+        a.num_nodes = pos_min.shape[0]
+        a.ray_origin, a.ray_direction, a.num_rays = origin.data_ptr(), direction.data_ptr(), R
+        a.ray_seg_offset = segs["ray_seg_offset"].data_ptr()
+        a.num_segments = K
+        for f in ("seg_node", "seg_ray", "seg_count", "seg_tmin", "seg_tmax"):
+            setattr(a, f, segs[f].data_ptr())
+        a.num_samples, a.jitter, a.seed = S, int(bool(jitter)), seed
+        a.grid, a.num_slots, a.channels, a.grid_size = grid.data_ptr(), N, C, G
+        for i, st in enumerate(grid.stride()):
+            a.grid_strides[i] = st
+        a.in_features, a.hidden, a.hidden_layers, a.out_features = mlp["in"], mlp["hidden"], mlp["layers"], mlp["out"]
+        a.params, a.act, a.last_act, a.softplus_beta = params.data_ptr(), _abi.ACT[mlp["act"]], \
+            _abi.ACT[mlp["last_act"]], mlp["beta"]
+        self.args = a
+        self.grid, self.params, self.R, self.S = grid, params, R, S
+        nbytes = _abi.lib().neat_projector_workspace_bytes(ctypes.byref(a))
+        if nbytes == 0:
+            raise _abi.NeatError(_abi.NEAT_E_UNSUPPORTED, "neat_projector_workspace_bytes", _abi.last_error())
+        self.ws = torch.empty(nbytes, dtype=torch.uint8, device=grid.device)
+        call("neat_projector_plan", stream_ptr(stream), ctypes.byref(a), ptr(self.ws), nbytes)
+
+    def forward(self, compensated=False):
+        proj = torch.empty(self.R, dtype=_F32, device=self.grid.device)
+        call("neat_projector_fwd", stream_ptr(self.stream), ctypes.byref(self.args), int(bool(compensated)), ptr(proj),
+             ptr(self.ws), self.ws.numel())
+        return proj
+
+    def backward(self, grad_proj):
+        grad_proj = grad_proj.contiguous()
+        gg = torch.empty_like(self.grid)
+        gp = torch.empty_like(self.params)
+        call("neat_projector_bwd", stream_ptr(self.stream), ctypes.byref(self.args), ptr(grad_proj), ptr(gg), ptr(gp),
+             ptr(self.ws), self.ws.numel())
+        return gg, gp

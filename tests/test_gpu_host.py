@@ -1,0 +1,130 @@
+"""The libtorch module layer (reference API: HyperTreeBase, IndirectGridSample3D, FCBlock, GeometryExEx::ComputeImage)
+on the GPU, against the oracle and plain PyTorch fp32 references, including autograd backward."""
+import numpy as np
+import pytest
+import torch
+
+from neat_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+@pytest.fixture(scope="module")
+def H():
+    import neat_amd.host as host
+    return host.load()
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def make_geometry(H, cfg, seed=0):
+    sc = W.make_scene(cfg, seed=seed)
+    tree = H.HyperTreeBase(3, cfg.depth)
+    tree.node_active.copy_(torch.from_numpy(sc["active"]))
+    tree.UpdateActive()
+    tree.to(DEV)
+    p = H.CombinedParams()
+    p.net_params.grid_size = cfg.grid
+    p.net_params.grid_features = cfg.channels
+    p.net_params.decoder_hidden_features = cfg.hidden
+    p.net_params.decoder_hidden_layers = cfg.layers
+    p.net_params.decoder_activation = cfg.act
+    p.net_params.softplus_beta = cfg.beta
+    p.octree_params.max_samples_per_node = cfg.max_spn
+    p.octree_params.tree_optimizer_params.max_active_nodes = sc["grid"].shape[0]
+    p.train_params.grid_init = "zero"
+    geo = H.GeometryExEx(1, 3, tree, p)
+    geo.to(DEV)
+    with torch.no_grad():
+        geo.grid.copy_(torch.from_numpy(sc["grid"]).to(DEV))
+    geo.decoder.load_packed(torch.from_numpy(sc["params"]).to(DEV))
+    return sc, tree, p, geo
+
+
+def test_tree_sampler_and_local_samples(H, oracle):
+    print("case: HyperTreeBase.CreateSamplesForRays C1", flush=True)
+    sc, tree, p, geo = make_geometry(H, W.C1)
+    o, d = W.batch_rays(W.C1, 4096, seed=3)
+    rays = H.RayList(torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV))
+    smp = tree.CreateSamplesForRays(rays, 32, False)
+    ref = oracle.raysample(sc["tree"], sc["active_ids"], o, d, 32)
+    assert np.array_equal(smp.node_id.cpu().numpy(), ref["node_id"])
+    assert np.array_equal(smp.ray_t.cpu().numpy().view(np.int32), ref["t"].view(np.int32))
+    assert np.array_equal(smp.weight.cpu().numpy().view(np.int32), ref["weight"].view(np.int32))
+    assert np.array_equal(smp.global_coordinate.cpu().numpy().view(np.int32), ref["coord"].view(np.int32))
+    assert smp.max_samples_per_ray == int(np.diff(ref["ray_sample_offset"]).max())
+    local = tree.ComputeLocalSamples(smp.global_coordinate, smp.node_id)
+    assert np.array_equal(local.cpu().numpy(), oracle.local_samples(sc["tree"], ref["coord"], ref["node_id"]))
+    slot = tree.GlobalNodeIdToLocalActiveId(smp.node_id)
+    assert np.array_equal(slot.cpu().numpy(), sc["prefix"][ref["node_id"]])
+
+
+def test_indirect_grid_sample_autograd_vs_torch(H):
+    print("case: IndirectGridSample3D autograd vs torch.grid_sample", flush=True)
+    rng = np.random.default_rng(0)
+    grid = torch.from_numpy((rng.normal(size=(6, 8, 9, 9, 9)) * 0.2).astype(np.float32)).to(DEV)
+    grid = grid.contiguous(memory_format=torch.channels_last_3d).requires_grad_(True)
+    idx = torch.from_numpy(rng.integers(0, 6, 5000)).to(DEV)
+    uv = torch.from_numpy(rng.uniform(-0.99, 0.99, (5000, 3)).astype(np.float32)).to(DEV).requires_grad_(True)
+    out = H.IndirectGridSample3D(grid, idx, uv)
+    go = torch.randn_like(out)
+    (out * go).sum().backward()
+    g2 = grid.detach().clone().requires_grad_(True)
+    uv2 = uv.detach().clone().requires_grad_(True)
+    ref = torch.nn.functional.grid_sample(g2[idx], uv2.view(-1, 1, 1, 1, 3), mode="bilinear", padding_mode="border",
+                                          align_corners=True).view(5000, 8)
+    (ref * go).sum().backward()
+    assert rel(out.detach().cpu(), ref.detach().cpu()) < 1e-6
+    assert rel(grid.grad.cpu(), g2.grad.cpu()) < 1e-5
+    assert rel(uv.grad.cpu(), uv2.grad.cpu()) < 1e-5
+
+
+def test_fcblock_matches_torch_linear(H):
+    print("case: FCBlock vs torch Linear chain", flush=True)
+    fc = H.FCBlock(8, 1, 1, 64, True, "silu")
+    fc.to(DEV)
+    x = torch.randn(3000, 8, device=DEV, requires_grad=True)
+    y = fc.forward_with_last(x, "softplus", 2.0)
+    params = dict(fc.named_parameters())
+    h = torch.nn.functional.silu(x @ params["seq.0.weight"].T + params["seq.0.bias"])
+    h = torch.nn.functional.silu(h @ params["seq.2.weight"].T + params["seq.2.bias"])
+    ref = torch.nn.functional.softplus(h @ params["seq.4.weight"].T + params["seq.4.bias"], beta=2.0)
+    assert rel(y.detach().cpu(), ref.detach().cpu()) < 1e-5
+    g = torch.randn_like(y)
+    gx = torch.autograd.grad((y * g).sum(), [x] + list(params.values()))
+    gref = torch.autograd.grad((ref * g).sum(), [x] + list(params.values()))
+    for a, b in zip(gx, gref):
+        assert rel(a.cpu(), b.cpu()) < 1e-5
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_compute_image_and_backward_vs_oracle(H, oracle, fused):
+    print("case: GeometryExEx.ComputeImage fused =", fused, flush=True)
+    sc, tree, p, geo = make_geometry(H, W.C1)
+    p.train_params.use_fused_projector = fused
+    o, d = W.batch_rays(W.C1, 5000, seed=4)
+    rays = H.RayList(torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV))
+    smp = tree.CreateSamplesForRays(rays, 32, True)
+    img = geo.ComputeImage(smp, 1, o.shape[0])
+    assert img.shape == (1, o.shape[0])
+    tgt = torch.from_numpy(sc["phantom"].line_integrals(o, d).astype(np.float32)).to(DEV)
+    loss = ((img - tgt.view(1, -1)) ** 2).mean()
+    loss.backward()
+    seg = smp.segments
+    hseg = dict(ray_seg_offset=seg.ray_seg_offset.cpu().numpy(), seg_node=seg.node_id.cpu().numpy(),
+                seg_ray=seg.ray.cpu().numpy(), seg_count=seg.count.cpu().numpy(), seg_tmin=seg.tmin.cpu().numpy(),
+                seg_tmax=seg.tmax.cpu().numpy())
+    mlp = dict(sc["mlp"], params=sc["params"])
+    ref = oracle.projector_fwd(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, jitter=True, seed=seg.seed)
+    assert rel(img.detach().cpu().numpy()[0], ref) < 1e-5
+    gproj = 2 * (ref - tgt.cpu().numpy()) / o.shape[0]
+    rgg, rgp = oracle.projector_bwd(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, gproj, jitter=True,
+                                    seed=seg.seed)
+    assert rel(geo.grid.grad.cpu().numpy(), rgg) < 1e-5
+    gp = torch.cat([t.grad.reshape(-1) for n, t in geo.named_parameters() if n.startswith("decoder")])
+    assert rel(gp.cpu().numpy(), rgp) < 1e-5

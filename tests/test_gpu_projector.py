@@ -1,0 +1,113 @@
+"""GPU parity of the fused projector (neat_projector_plan / _fwd / _bwd, through the C ABI) against the CPU oracle
+and against the unfused kernel chain, with the error against the FP64 oracle reported.
+
+Bar: FP32 projections and grid / decoder gradients within 1e-5 relative of the FP32 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from neat_amd import ops, pipeline, workloads as W
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+RTOL = 1e-5
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def setup(cfg, nrays, seed=0, hidden=None, act=None):
+    sc = W.make_scene(cfg, seed=seed)
+    if hidden is not None or act is not None:
+        h = hidden or cfg.hidden
+        sc["params"] = W.decoder_params(cfg.channels, h, cfg.layers, 1, seed + 2)
+        sc["mlp"] = dict(sc["mlp"], hidden=h, act=act or sc["mlp"]["act"])
+    ds = pipeline.to_device(sc, cfg)
+    o, d = W.batch_rays(cfg, nrays, seed=seed + 7)
+    return sc, ds, o, d, torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV)
+
+
+def run_fused(ds, to, td, segs, jitter, seed, grad=None):
+    pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, to, td, segs, ds.grid, ds.params, ds.mlp, jitter, seed)
+    proj = pj.forward()
+    out = [proj]
+    if grad is not None:
+        out += list(pj.backward(grad))
+    torch.cuda.synchronize()
+    return out
+
+
+@pytest.mark.parametrize("jitter", [False, True])
+@pytest.mark.parametrize("variant", [dict(), dict(hidden=64, act="silu")])
+def test_fused_projector_matches_oracle_uniform_tree(oracle, jitter, variant):
+    print("case: fused projector C1 geometry", jitter, variant, flush=True)
+    cfg = W.C1
+    sc, ds, o, d, to, td = setup(cfg, 6000, **variant)
+    segs = pipeline.sample(ds, to, td, jitter=jitter, seed=3, emit=False)
+    rng = np.random.default_rng(1)
+    gproj = rng.normal(size=o.shape[0]).astype(np.float32)
+    proj, gg, gp = run_fused(ds, to, td, segs, jitter, 3, torch.from_numpy(gproj).to(DEV))
+    hseg = {k: v.cpu().numpy() for k, v in segs.items() if torch.is_tensor(v)}
+    mlp = dict(sc["mlp"], params=sc["params"])
+    ref = oracle.projector_fwd(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, jitter=jitter, seed=3, prec=0)
+    assert rel(proj.cpu().numpy(), ref) <= RTOL
+    ref64 = oracle.projector_fwd(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, jitter=jitter, seed=3, prec=1)
+    print("fp32 fused vs fp64 oracle: rel", rel(proj.cpu().numpy(), ref64))
+    rgg, rgp = oracle.projector_bwd(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, gproj, jitter=jitter,
+                                    seed=3, prec=0)
+    assert rel(gg.cpu().numpy(), rgg) <= RTOL
+    assert rel(gp.cpu().numpy(), rgp) <= RTOL
+    assert gg.stride() == ds.grid.stride()
+
+
+def test_fused_projector_matches_unfused_chain():
+    print("case: fused vs unfused, C1 geometry", flush=True)
+    cfg = W.C1
+    sc, ds, o, d, to, td = setup(cfg, 20000)
+    tgt = torch.from_numpy(sc["phantom"].line_integrals(o, d).astype(np.float32)).to(DEV)
+    loss, proj_u, gg_u, gp_u, S = pipeline.unfused_step(ds, to, td, tgt, jitter=True, seed=9)
+    segs = pipeline.sample(ds, to, td, jitter=True, seed=9, emit=False)
+    pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, to, td, segs, ds.grid, ds.params, ds.mlp, True, 9)
+    proj = pj.forward()
+    loss2, gproj = ops.mse_fp64(proj, tgt)
+    gg, gp = pj.backward(gproj)
+    assert rel(proj.cpu().numpy(), proj_u.reshape(-1).cpu().numpy()) <= RTOL
+    assert abs(loss.item() - loss2.item()) <= 1e-5 * loss.item()
+    assert rel(gg.cpu().numpy(), gg_u.cpu().numpy()) <= RTOL
+    assert rel(gp.cpu().numpy(), gp_u.cpu().numpy()) <= RTOL
+
+
+def test_fused_projector_adaptive_tree(oracle):
+    print("case: fused projector C2 geometry, 768 rays", flush=True)
+    cfg = W.C2
+    sc, ds, o, d, to, td = setup(cfg, 768)
+    segs = pipeline.sample(ds, to, td, jitter=True, seed=4, emit=False)
+    gproj = np.random.default_rng(2).normal(size=o.shape[0]).astype(np.float32)
+    proj, gg, gp = run_fused(ds, to, td, segs, True, 4, torch.from_numpy(gproj).to(DEV))
+    hseg = {k: v.cpu().numpy() for k, v in segs.items() if torch.is_tensor(v)}
+    mlp = dict(sc["mlp"], params=sc["params"])
+    ref = oracle.projector_fwd(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, jitter=True, seed=4, prec=0)
+    assert rel(proj.cpu().numpy(), ref) <= RTOL
+    rgg, rgp = oracle.projector_bwd(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, gproj, jitter=True,
+                                    seed=4, prec=0, nthreads=1)
+    assert rel(gg.cpu().numpy(), rgg) <= RTOL
+    assert rel(gp.cpu().numpy(), rgp) <= RTOL
+
+
+def test_fused_projector_deterministic_forward_and_empty():
+    print("case: fused determinism + empty batch", flush=True)
+    cfg = W.C1
+    sc, ds, o, d, to, td = setup(cfg, 4000)
+    segs = pipeline.sample(ds, to, td, jitter=True, seed=5, emit=False)
+    p1 = run_fused(ds, to, td, segs, True, 5)[0]
+    p2 = run_fused(ds, to, td, segs, True, 5)[0]
+    assert torch.equal(p1, p2)
+    # rays that miss everything: zero projection and zero gradients
+    far_o = torch.full((16, 3), 5.0, device=DEV)
+    far_d = torch.tensor([[1.0, 0.0, 0.0]], device=DEV).repeat(16, 1)
+    segs0 = pipeline.sample(ds, far_o, far_d, emit=False)
+    proj, gg, gp = run_fused(ds, far_o, far_d, segs0, False, 0, torch.ones(16, device=DEV))
+    assert torch.all(proj == 0) and torch.all(gg == 0) and torch.all(gp == 0)
