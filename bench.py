@@ -30,6 +30,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--path", default="fused", choices=["fused", "unfused"])
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32", "fp32_valu"],
+                    help="fused path arithmetic: BF16 on the matrix cores (default), FP32 on the matrix cores, or "
+                         "FP32 on the vector ALUs")
     ap.add_argument("--batches", type=int, default=4, help="distinct resident ray batches cycled by the steps")
     ap.add_argument("--cpu-rays", type=int, default=2048, help="rays in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -57,6 +60,8 @@ def main():
         batches.append((torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev), torch.from_numpy(tgt).to(dev)))
     torch.cuda.synchronize()
 
+    # the BF16 grid copy is refreshed once per optimizer step in training; here the grid is fixed, so once
+    grid_bf16 = ops.grid_to_bf16(ds.grid) if (a.path == "fused" and a.precision == "bf16") else None
     stage_names = (["sample", "plan", "proj_fwd", "loss", "proj_bwd"] if a.path == "fused" else
                    ["sample", "gs_fwd", "dec_fwd", "integrate", "dec_bwd", "gs_bwd"])
     events = {k: [] for k in stage_names}
@@ -69,7 +74,8 @@ def main():
         segs = pipeline.sample(ds, o, d, jitter=True, seed=i, emit=False)
         rec("sample", 1)
         rec("plan", 0)
-        pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, o, d, segs, ds.grid, ds.params, ds.mlp, True, i)
+        pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, o, d, segs, ds.grid, ds.params, ds.mlp, True, i,
+                           precision=a.precision, grid_bf16=grid_bf16)
         rec("plan", 1)
         rec("proj_fwd", 0)
         proj = pj.forward()
@@ -136,10 +142,11 @@ def main():
     dominant = max(stage_ms, key=stage_ms.get)
     C, H = cfg.channels, cfg.hidden
     fwd_flop = 2 * (C * H + H * H + H * 1)  # per sample, 2 FLOP per MAC
+    mm_peak = BF16_PEAK_TFLOPS if (a.path == "fused" and a.precision == "bf16") else FP32_PEAK_TFLOPS
     # algorithmic work per launch of each stage (per-sample figure x samples of the batch)
     work = {
-        "proj_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS),
-        "proj_bwd": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS),
+        "proj_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", mm_peak),
+        "proj_bwd": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", mm_peak),
         "plan": ("hbm", 16 * mean_samples / 27, "GB/s", HBM_PEAK_GBS),
         "loss": ("hbm", 12 * cfg.batch, "GB/s", HBM_PEAK_GBS),
         "dec_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS),
@@ -160,6 +167,7 @@ def main():
     if not a.no_cpu:
         cpu = cpu_baseline(scene, cfg, a.cpu_rays, a.seed)
 
+    prec_name = a.precision if a.path == "fused" else "fp32_unfused"
     out = {
         "metric": "fwd+bwd rays/s at 1 GPU, BF16+proposal vs plain FP32, at <=1e-3 rel err vs FP64",
         "value": round(rays_per_s, 1),
@@ -171,11 +179,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": prec_name.split("_")[0],
         "data": "synthetic",
-        "config": {"workload": "C2 plain FP32 projector fwd+bwd: depth-5 octree, 4096 adaptive leaves, 17^3x8 grids, "
+        "config": {"workload": f"projector fwd+bwd ({prec_name}): depth-5 octree, 4096 adaptive leaves, 17^3x8 grids, "
                                "decoder 8->64->64->1, 256x512^2 cone beam, batch 2^18 rays, max 64 samples/node",
-                   "path": a.path, "batch_rays": cfg.batch, "samples_per_batch": round(mean_samples),
+                   "path": a.path, "precision": prec_name, "batch_rays": cfg.batch, "samples_per_batch": round(mean_samples),
                    "samples_per_s": round(mean_samples / (ms_per_step / 1e3), 1),
                    "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}},
         "roofline": roofline,

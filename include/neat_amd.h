@@ -228,7 +228,19 @@ typedef struct neat_projector_args
     int32_t act;
     int32_t last_act;
     float softplus_beta;
+    /* arithmetic: NEAT_PREC_FP32 (default), NEAT_PREC_BF16 (grid read from grid_bf16, decoder weights rounded to BF16,
+     * FP32 accumulation), NEAT_PREC_FP32_VALU (the first FP32 kernels on the vector ALUs, kept for comparison) */
+    int32_t precision;
+    /* BF16 copy of the grid, same shape and channels-last layout (NEAT_PREC_BF16 only; see neat_grid_to_bf16) */
+    const uint16_t* grid_bf16;
 } neat_projector_args;
+
+#define NEAT_PREC_FP32 0
+#define NEAT_PREC_BF16 1
+#define NEAT_PREC_FP32_VALU 2
+
+/* dst[i] = round-to-nearest-even BF16 of src[i] (the BF16 grid copy the NEAT_PREC_BF16 projector reads). */
+int neat_grid_to_bf16(neat_stream_t stream, const float* src, int64_t n, uint16_t* dst);
 
 /* Workspace for plan + forward + backward of these arguments (depends on the counts, not on the data). */
 size_t neat_projector_workspace_bytes(const neat_projector_args* args);

@@ -63,7 +63,11 @@ class ProjectorArgs(ctypes.Structure):
         ("grid", _P), ("num_slots", _I64), ("channels", _I32), ("grid_size", _I32), ("grid_strides", _I64 * 5),
         ("in_features", _I32), ("hidden", _I32), ("hidden_layers", _I32), ("out_features", _I32),
         ("params", _P), ("act", _I32), ("last_act", _I32), ("softplus_beta", _F),
+        ("precision", _I32), ("grid_bf16", _P),
     ]
+
+
+PREC = {"fp32": 0, "bf16": 1, "fp32_valu": 2}
 
 
 _PA = ctypes.POINTER(ProjectorArgs)
@@ -72,6 +76,7 @@ SIGNATURES.update({
     "neat_projector_plan": (ctypes.c_int, [_P, _PA, _P, _SZ]),
     "neat_projector_fwd": (ctypes.c_int, [_P, _PA, _I32, _P, _P, _SZ]),
     "neat_projector_bwd": (ctypes.c_int, [_P, _PA, _P, _P, _P, _P, _SZ]),
+    "neat_grid_to_bf16": (ctypes.c_int, [_P, _P, _I64, _P]),
 })
 
 _LIB = None

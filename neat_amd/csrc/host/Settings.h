@@ -37,6 +37,9 @@ struct MyTrainParams
     // projector execution: the fused kernels when the configuration is covered, compensated per-ray sums
     bool use_fused_projector = true;
     bool compensated_sum     = false;
+    // fused projector arithmetic: "fp32" (matrix cores), "bf16" (BF16 grid copy and hidden layers, FP32
+    // accumulation) or "fp32_valu" (vector ALUs)
+    std::string projector_precision = "fp32";
 };
 
 struct CombinedParams

@@ -137,7 +137,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def(py::init<>())
         .def_readwrite("grid_init", &MyTrainParams::grid_init)
         .def_readwrite("use_fused_projector", &MyTrainParams::use_fused_projector)
-        .def_readwrite("compensated_sum", &MyTrainParams::compensated_sum);
+        .def_readwrite("compensated_sum", &MyTrainParams::compensated_sum)
+        .def_readwrite("projector_precision", &MyTrainParams::projector_precision);
     py::class_<CombinedParams, std::shared_ptr<CombinedParams>>(m, "CombinedParams")
         .def(py::init<>())
         .def_readwrite("net_params", &CombinedParams::net_params)

@@ -54,7 +54,8 @@ struct ProjectorFn : public torch::autograd::Function<ProjectorFn>
     {
         // t: grid, params, pmin, pmax, prefix, origin, direction, ray_seg_offset, seg_node, seg_ray, seg_count,
         //    seg_tmin, seg_tmax
-        // iv: num_samples, jitter, seed, in, hidden, layers, out, act, last_act
+        // iv: num_samples, jitter, seed, in, hidden, layers, out, act, last_act, precision
+        // t[13]: BF16 copy of the grid (NEAT_PREC_BF16 only)
         neat_projector_args a{};
         a.node_pos_min           = dptr<float>(t[2]);
         a.node_pos_max           = dptr<float>(t[3]);
@@ -86,6 +87,8 @@ struct ProjectorFn : public torch::autograd::Function<ProjectorFn>
         a.act           = (int32_t)iv[7];
         a.last_act      = (int32_t)iv[8];
         a.softplus_beta = (float)beta;
+        a.precision     = (int32_t)iv[9];
+        a.grid_bf16     = a.precision == NEAT_PREC_BF16 ? reinterpret_cast<const uint16_t*>(dptr<int16_t>(t[13])) : nullptr;
         return a;
     }
 
@@ -94,11 +97,22 @@ struct ProjectorFn : public torch::autograd::Function<ProjectorFn>
     {
         std::vector<torch::Tensor> t = {grid, params.contiguous()};
         for (auto& g : geo) t.push_back(g);
+        auto s = current_stream();
+        if (iv[9] == NEAT_PREC_BF16)
+        {
+            // the grid changes every optimizer step: round it once per forward
+            auto gb = torch::empty_like(grid, grid.options().dtype(torch::kInt16));
+            NEAT_CALL(neat_grid_to_bf16(s, dptr<float>(grid), grid.numel(), reinterpret_cast<uint16_t*>(dptr<int16_t>(gb))));
+            t.push_back(gb);
+        }
+        else
+        {
+            t.push_back(torch::empty({0}, grid.options().dtype(torch::kInt16)));
+        }
         neat_projector_args a = args(t, iv, beta);
         const size_t bytes    = neat_projector_workspace_bytes(&a);
         TORCH_CHECK(bytes > 0, "fused projector: ", neat_last_error());
         auto ws   = torch::empty({(int64_t)bytes}, grid.options().dtype(torch::kUInt8));
-        auto s    = current_stream();
         auto proj = torch::empty({1, a.num_rays}, grid.options().memory_format(at::MemoryFormat::Contiguous));
         NEAT_CALL(neat_projector_plan(s, &a, ws.data_ptr(), bytes));
         NEAT_CALL(neat_projector_fwd(s, &a, compensated ? 1 : 0, dptr<float>(proj), ws.data_ptr(), bytes));
@@ -217,6 +231,9 @@ std::string GeometryExEx::FusedUnsupportedReason()
     if (!(np.decoder_activation == "relu" || np.decoder_activation == "silu")) return "hidden activation not relu/silu";
     if (np.last_activation_function != "softplus") return "last activation is not softplus";
     if (num_channels != 1) return "num_channels != 1";
+    const int prec = precision_id(params->train_params.projector_precision);
+    if (prec < 0) return "projector_precision not in {fp32, bf16, fp32_valu}";
+    if (prec == NEAT_PREC_BF16 && C != 8) return "the bf16 projector needs grid_features == 8";
     return "";
 }
 
@@ -231,7 +248,7 @@ torch::Tensor GeometryExEx::ComputeImageFused(const SampleList& samples, int num
                                       s.tmin, s.tmax};
     std::vector<int64_t> iv = {s.num_samples, s.jitter ? 1 : 0, (int64_t)s.seed, np.grid_features,
                                np.decoder_hidden_features, np.decoder_hidden_layers, 1, act_id(np.decoder_activation),
-                               NEAT_ACT_SOFTPLUS};
+                               NEAT_ACT_SOFTPLUS, precision_id(params->train_params.projector_precision)};
     return ProjectorFn::apply(explicit_grid_generator->grid_data, decoder->packed_parameters(), geo, iv,
                               (double)np.softplus_beta, params->train_params.compensated_sum);
 }

@@ -33,6 +33,15 @@ inline int act_id(const std::string& name)
     return -1;
 }
 
+// projector arithmetic by name; -1 for an unknown name
+inline int precision_id(const std::string& name)
+{
+    if (name == "fp32") return NEAT_PREC_FP32;
+    if (name == "bf16") return NEAT_PREC_BF16;
+    if (name == "fp32_valu") return NEAT_PREC_FP32_VALU;
+    return -1;
+}
+
 template <typename T>
 inline T* dptr(const torch::Tensor& t)
 {

@@ -1,0 +1,259 @@
+// Wave-level decoder MLP on CDNA4 matrix cores, for tiles of 32 samples per wave.
+//
+// The MLP is evaluated transposed, H^T = W X^T: the M dimension is the hidden width (one or two 32-row tiles), N is
+// the tile's 32 samples, K the layer input. With 32x32 MFMA shapes, lane l of the wave owns sample n = l % 32 and
+// lane group g = l / 32; the accumulator register i of lane l holds row crow(i, g) = (i/4)*8 + g*4 + i%4 of column
+// n. The B operand of the next layer needs, per lane, the K slots (l/32)*KS + j of column n: taking the K order of
+// the next layer to be "register i of group g" makes the accumulator registers themselves the B operand, so layers
+// chain in registers. The weight fragments are laid out with the same K permutation once per workgroup, in LDS.
+//
+// Precisions: FP32 on v_mfma_f32_32x32x2f32 (KS = 1 slot per lane per step); BF16 on v_mfma_f32_32x32x8bf16_1k for
+// the 8-channel first layer and v_mfma_f32_32x32x16_bf16 (KS = 8) for hidden layers, FP32 accumulation.
+// Lane layouts are those of the instructions (A: row l%32, K slots (l/32)*KS..; B: K slots (l/32)*KS.., column l%32;
+// C: crow(i, l/32), column l%32).
+#pragma once
+
+#include "common.hpp"
+
+namespace neat
+{
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ int crow(int i, int g) { return ((i >> 2) << 3) + (g << 2) + (i & 3); }
+
+__device__ __forceinline__ short bf16_bits(float f)
+{
+    // round to nearest even (no NaN inputs on this path)
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_to_f32(unsigned short b) { return __uint_as_float(((uint32_t)b) << 16); }
+
+struct PrecF32
+{
+    static constexpr int KS = 1;   // K slots per lane per hidden-layer step
+    static constexpr bool kBF16 = false;
+    typedef float Frag;            // hidden-layer A/B fragment
+    typedef float Frag1;           // first-layer fragment (one channel per step, 4 steps)
+    static constexpr int kSteps1 = 4;
+    __device__ static __forceinline__ f32x16 mma(Frag a, Frag b, f32x16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ f32x16 mma1(Frag1 a, Frag1 b, f32x16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+    }
+    // B fragment of step st from accumulator registers (after an elementwise map f)
+    template <typename F>
+    __device__ static __forceinline__ Frag from_acc(const f32x16& z, int st, F&& f)
+    {
+        return f(z[st], st);
+    }
+};
+
+struct PrecBF16
+{
+    static constexpr int KS = 8;
+    static constexpr bool kBF16 = true;
+    typedef bf16x8 Frag;
+    typedef s16x4 Frag1;           // 4 channels per lane, one K = 8 step
+    static constexpr int kSteps1 = 1;
+    __device__ static __forceinline__ f32x16 mma(Frag a, Frag b, f32x16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ f32x16 mma1(Frag1 a, Frag1 b, f32x16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+    }
+    template <typename F>
+    __device__ static __forceinline__ Frag from_acc(const f32x16& z, int st, F&& f)
+    {
+        Frag b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[j] = (__bf16)f(z[st * 8 + j], st * 8 + j);
+        return b;
+    }
+};
+
+// Steps per 32-row source tile of a hidden layer.
+template <typename P>
+constexpr int steps_per_tile()
+{
+    return 16 / P::KS;
+}
+
+// ---- weight fragments in LDS -------------------------------------------------------------------------------------
+// Layout (per workgroup), all indexed [...][lane]:
+//   W1f [MT][kSteps1][64]         first layer, dst row 32mt + l%32, channel (l/32)*4 + step (FP32) or + j (BF16)
+//   W2f [MT][MT][S][64]           hidden layer, dst row 32mt + l%32, src row 32ms + crow(st*KS + j, l/32)
+//   W2Tf[MT][MT][S][64]           transposed: dst = layer-1 row 32mt + l%32, src = layer-2 row 32ms + crow(...)
+//   W1Tf[MT][S][64]               dX: dst channel l%32 (< C, else 0), src = layer-1 row 32ms + crow(...)
+//   vec  [4][64]                  b1, b2, w3 (fp32), b3 at vec[3][0]
+template <typename P, int H>
+struct WeightLDS
+{
+    static constexpr int MT = H / 32;
+    static constexpr int S  = steps_per_tile<P>();
+    typename P::Frag1 w1[MT][P::kSteps1][64];
+    typename P::Frag w2[MT][MT][S][64];
+    typename P::Frag w2t[MT][MT][S][64];
+    typename P::Frag w1t[MT][S][64];
+    float vec[4][64];
+};
+
+// Packed parameter layout (C -> H -> H -> 1): W0 [H][C], b0 [H], W1 [H][H], b1 [H], W2 [1][H], b2 [1].
+template <typename P, int H, int C>
+__device__ void load_weights(const float* __restrict__ params, WeightLDS<P, H>& w)
+{
+    constexpr int MT = H / 32, S = steps_per_tile<P>();
+    const float* W0 = params;
+    const float* b0 = W0 + H * C;
+    const float* W1 = b0 + H;
+    const float* b1 = W1 + H * H;
+    const float* W2 = b1 + H;
+    const float* b2 = W2 + H;
+    for (int t = threadIdx.x; t < MT * 64; t += blockDim.x)
+    {
+        const int mt = t / 64, l = t % 64, g = l >> 5, r = 32 * mt + (l & 31);
+        if constexpr (P::kBF16)
+        {
+            s16x4 f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) f[j] = bf16_bits(W0[r * C + 4 * g + j]);
+            w.w1[mt][0][l] = f;
+        }
+        else
+        {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) w.w1[mt][s][l] = W0[r * C + 4 * g + s];
+        }
+    }
+    for (int t = threadIdx.x; t < MT * MT * S * 64; t += blockDim.x)
+    {
+        const int l = t % 64, st = (t / 64) % S, ms = (t / (64 * S)) % MT, mt = t / (64 * S * MT);
+        const int g = l >> 5, dst = 32 * mt + (l & 31);
+        typename P::Frag a, b;
+#pragma unroll
+        for (int j = 0; j < P::KS; ++j)
+        {
+            const int src = 32 * ms + crow(st * P::KS + j, g);
+            float va      = W1[dst * H + src];  // forward: W1[dst][src]
+            float vb      = W1[src * H + dst];  // transposed: W1^T[dst][src] = W1[src][dst]
+            if constexpr (P::kBF16)
+            {
+                a[j] = (__bf16)va;
+                b[j] = (__bf16)vb;
+            }
+            else
+            {
+                a = va;
+                b = vb;
+            }
+        }
+        w.w2[mt][ms][st][l]  = a;
+        w.w2t[mt][ms][st][l] = b;
+    }
+    for (int t = threadIdx.x; t < MT * S * 64; t += blockDim.x)
+    {
+        const int l = t % 64, st = (t / 64) % S, ms = t / (64 * S);
+        const int g = l >> 5, c = l & 31;
+        typename P::Frag f;
+#pragma unroll
+        for (int j = 0; j < P::KS; ++j)
+        {
+            const int src = 32 * ms + crow(st * P::KS + j, g);
+            float v       = c < C ? W0[src * C + c] : 0.f;
+            if constexpr (P::kBF16)
+                f[j] = (__bf16)v;
+            else
+                f = v;
+        }
+        w.w1t[ms][st][l] = f;
+    }
+    for (int t = threadIdx.x; t < 64; t += blockDim.x)
+    {
+        w.vec[0][t] = t < H ? b0[t] : 0.f;
+        w.vec[1][t] = t < H ? b1[t] : 0.f;
+        w.vec[2][t] = t < H ? W2[t] : 0.f;
+        w.vec[3][t] = t == 0 ? b2[0] : 0.f;
+    }
+}
+
+// Per-tile forward state of a lane: pre-activations of both hidden layers (C layout) and the output pre-activation.
+template <int MT>
+struct Fwd
+{
+    f32x16 z1[MT];
+    f32x16 z2[MT];
+    float z;
+};
+
+// First layer input: the lane's 4 channels (group g holds channels 4g..4g+3 of its sample).
+template <typename P>
+__device__ __forceinline__ typename P::Frag1 x_step(const float (&x4)[4], int s)
+{
+    if constexpr (P::kBF16)
+    {
+        s16x4 f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) f[j] = bf16_bits(x4[j]);
+        return f;
+    }
+    else
+    {
+        return x4[s];
+    }
+}
+
+template <typename P, int H, int ACT>
+__device__ __forceinline__ void tile_forward_mfma(const WeightLDS<P, H>& w, const float (&x4)[4], float beta,
+                                                  Fwd<H / 32>& f)
+{
+    constexpr int MT = H / 32, S = steps_per_tile<P>();
+    const int l = threadIdx.x & 63, g = l >> 5;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+    {
+        f32x16 acc = {};
+#pragma unroll
+        for (int s = 0; s < P::kSteps1; ++s) acc = P::mma1(w.w1[mt][s][l], x_step<P>(x4, s), acc);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[i] += w.vec[0][32 * mt + crow(i, g)];
+        f.z1[mt] = acc;
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+    {
+        f32x16 acc = {};
+#pragma unroll
+        for (int ms = 0; ms < MT; ++ms)
+        {
+#pragma unroll
+            for (int st = 0; st < S; ++st)
+            {
+                auto b = P::from_acc(f.z1[ms], st, [&](float v, int) { return act_f(ACT, v, beta); });
+                acc    = P::mma(w.w2[mt][ms][st][l], b, acc);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[i] += w.vec[1][32 * mt + crow(i, g)];
+        f.z2[mt] = acc;
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+    {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) part = fmaf(w.vec[2][32 * mt + crow(i, g)], act_f(ACT, f.z2[mt][i], beta), part);
+    }
+    f.z = part + __shfl_xor(part, 32) + w.vec[3][0];
+}
+
+}  // namespace neat

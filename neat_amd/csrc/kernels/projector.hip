@@ -17,6 +17,7 @@
 //              atomics anywhere.
 #include <hipcub/hipcub.hpp>
 
+#include "mfma_mlp.hpp"
 #include "mlp_tile.hpp"
 
 namespace neat
@@ -494,6 +495,318 @@ __global__ void __launch_bounds__(256) k_fill_f32(float* p, int64_t n, float v)
     if (i < n) p[i] = v;
 }
 
+// ---- matrix-core path ---------------------------------------------------------------------------------------------------
+// Gather of the lane's 4 channels (group g: channels 4g..4g+3) of an 8-channel node grid, FP32 or BF16 storage,
+// trilinear weights and accumulation in FP32 (same corner order as the reference).
+template <bool kBF16Grid>
+__device__ __forceinline__ void gather_half(const void* node_grid, const Corners& c, int g, float (&x4)[4])
+{
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+    {
+        float4 v;
+        if constexpr (kBF16Grid)
+        {
+            uint2 raw = *reinterpret_cast<const uint2*>(static_cast<const unsigned short*>(node_grid) + c.off[k] + 4 * g);
+            v = make_float4(__uint_as_float(raw.x << 16), __uint_as_float(raw.x & 0xFFFF0000u),
+                            __uint_as_float(raw.y << 16), __uint_as_float(raw.y & 0xFFFF0000u));
+        }
+        else
+        {
+            v = *reinterpret_cast<const float4*>(static_cast<const float*>(node_grid) + c.off[k] + 4 * g);
+        }
+        acc.x += v.x * c.w[k];
+        acc.y += v.y * c.w[k];
+        acc.z += v.z * c.w[k];
+        acc.w += v.w * c.w[k];
+    }
+    x4[0] = acc.x;
+    x4[1] = acc.y;
+    x4[2] = acc.z;
+    x4[3] = acc.w;
+}
+
+__device__ __forceinline__ const void* node_grid_ptr(const ProjGeom& g, const void* grid_lowp, bool bf16, int32_t slot)
+{
+    const int64_t off = (int64_t)slot * g.G * g.G * g.G * 8;
+    return bf16 ? static_cast<const void*>(static_cast<const unsigned short*>(grid_lowp) + off)
+                : static_cast<const void*>(g.grid + off);
+}
+
+// Forward: per chunk, tiles of 32 samples per wave on the matrix cores, per-segment sums as in k_proj_fwd.
+template <typename P, int H, int ACT>
+__global__ void __launch_bounds__(256) k_mfma_fwd(ProjGeom g, Plan p, const float* __restrict__ params,
+                                                  const void* grid_lowp, float* seg_partial)
+{
+    __shared__ WeightLDS<P, H> w;
+    __shared__ float s_contrib[kFwdChunk];
+    __shared__ float s_seg[kFwdChunk];
+    const int64_t c = blockIdx.x;
+    if (c >= p.chunk_off1[g.N]) return;
+    load_weights<P, H, 8>(params, w);
+    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
+    const int32_t slot = p.chunk1_slot[c];
+    const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c];
+    const int64_t base  = p.nm_soff[q0];
+    const int64_t nsamp = p.nm_soff[q1] - base;
+    const void* ngrid   = node_grid_ptr(g, grid_lowp, P::kBF16, slot);
+    for (int32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) s_seg[q - q0] = 0.f;
+    __syncthreads();
+    for (int64_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
+    {
+        const int32_t wn = (int32_t)min<int64_t>(kFwdChunk, nsamp - w0);
+        for (int32_t t0 = wave * 32; t0 < wn; t0 += 128)
+        {
+            const int32_t i  = t0 + n;
+            const bool valid = i < wn;
+            float x4[4]      = {0.f, 0.f, 0.f, 0.f};
+            float wgt        = 0.f;
+            if (valid)
+            {
+                const int64_t s = w0 + i;
+                const int32_t q = find_segment(p.nm_soff, q0, q1, base + s);
+                Sample smp      = sample_geo(g, p.perm[q], (int32_t)(base + s - p.nm_soff[q]));
+                Corners cr      = corners_cl(smp.uv, g.G, 8);
+                gather_half<P::kBF16>(ngrid, cr, hg, x4);
+                wgt = smp.w;
+            }
+            Fwd<H / 32> f;
+            tile_forward_mfma<P, H, ACT>(w, x4, g.beta, f);
+            if (valid && hg == 0) s_contrib[i] = act_f(NEAT_ACT_SOFTPLUS, f.z, g.beta) * wgt;
+        }
+        __syncthreads();
+        for (int32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x)
+        {
+            const int64_t a = max<int64_t>(p.nm_soff[q] - base, w0);
+            const int64_t b = min<int64_t>(p.nm_soff[q + 1] - base, w0 + wn);
+            if (a >= b) continue;
+            float acc = 0.f;
+            for (int64_t s = a; s < b; ++s) acc += s_contrib[s - w0];
+            s_seg[q - q0] += acc;
+        }
+        __syncthreads();
+    }
+    for (int32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) seg_partial[p.perm[q]] = s_seg[q - q0];
+}
+
+__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// A / B fragment of a weight-gradient product over the tile's samples, from a [row][sample] staging buffer: FP32 step
+// st covers samples 2st + l/32; BF16 step st covers samples 16st + 8(l/32) + j.
+template <typename P>
+__device__ __forceinline__ typename P::Frag stage_frag(const float* buf, int row, int st, int hg, bool zero)
+{
+    typename P::Frag f;
+    if constexpr (P::kBF16)
+    {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) f[j] = (__bf16)(zero ? 0.f : buf[row * 33 + 16 * st + 8 * hg + j]);
+    }
+    else
+    {
+        f = zero ? 0.f : buf[row * 33 + 2 * st + hg];
+    }
+    return f;
+}
+
+// Backward: per wave and tile, recompute the forward, back-propagate on the matrix cores, write dL/dfeature per
+// sample (for the LDS scatter) and accumulate the parameter gradient; every wave writes its own slice of the
+// partials (summed later in a fixed order).
+template <typename P, int H, int ACT>
+__global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const float* __restrict__ params,
+                                                  const void* grid_lowp, const float* grad_proj, float* dfeat,
+                                                  float* partials, int32_t P_total)
+{
+    constexpr int MT = H / 32, S = steps_per_tile<P>(), C = 8;
+    constexpr int KSTEPS = 32 / (2 * P::KS);  // sample steps of a weight-gradient product (FP32 16, BF16 2)
+    __shared__ WeightLDS<P, H> w;
+    __shared__ float stage[4][2][H * 33];
+    load_weights<P, H, C>(params, w);
+    __syncthreads();
+    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
+    float* sA = stage[wave][0];
+    float* sB = stage[wave][1];
+    f32x16 dW2[MT][MT], dW1[MT];
+    f32x16 accb1[MT], accb2[MT], accw3[MT];
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+    {
+        dW1[a] = accb1[a] = accb2[a] = accw3[a] = f32x16{};
+#pragma unroll
+        for (int b = 0; b < MT; ++b) dW2[a][b] = f32x16{};
+    }
+    float accb3 = 0.f;
+    const int64_t nchunks = p.chunk_off1[g.N];
+    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x)
+    {
+        const int32_t slot = p.chunk1_slot[c];
+        const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c];
+        const int64_t base  = p.nm_soff[q0];
+        const int64_t nsamp = p.nm_soff[q1] - base;
+        const void* ngrid   = node_grid_ptr(g, grid_lowp, P::kBF16, slot);
+        for (int64_t t0 = wave * 32; t0 < nsamp; t0 += 128)
+        {
+            const int64_t s  = t0 + n;
+            const bool valid = s < nsamp;
+            float x4[4]      = {0.f, 0.f, 0.f, 0.f};
+            float gy         = 0.f;
+            if (valid)
+            {
+                const int32_t q = find_segment(p.nm_soff, q0, q1, base + s);
+                Sample smp      = sample_geo(g, p.perm[q], (int32_t)(base + s - p.nm_soff[q]));
+                Corners cr      = corners_cl(smp.uv, g.G, C);
+                gather_half<P::kBF16>(ngrid, cr, hg, x4);
+                gy = grad_proj[smp.ray] * smp.w;
+            }
+            Fwd<MT> f;
+            tile_forward_mfma<P, H, ACT>(w, x4, g.beta, f);
+            const float dz = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, f.z, g.beta) : 0.f;
+            if (hg == 0) accb3 += dz;
+            f32x16 dZ2[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+            {
+#pragma unroll
+                for (int i = 0; i < 16; ++i)
+                {
+                    const int r    = 32 * mt + crow(i, hg);
+                    const float zz = f.z2[mt][i];
+                    dZ2[mt][i]     = w.vec[2][r] * dz * act_d(ACT, zz, g.beta);
+                    accw3[mt][i] += dz * act_f(ACT, zz, g.beta);
+                    accb2[mt][i] += dZ2[mt][i];
+                }
+            }
+            f32x16 dZ1[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+            {
+                f32x16 acc = {};
+#pragma unroll
+                for (int ms = 0; ms < MT; ++ms)
+                {
+#pragma unroll
+                    for (int st = 0; st < S; ++st)
+                        acc = P::mma(w.w2t[mt][ms][st][l], P::from_acc(dZ2[ms], st, [](float v, int) { return v; }),
+                                     acc);
+                }
+#pragma unroll
+                for (int i = 0; i < 16; ++i)
+                {
+                    acc[i] *= act_d(ACT, f.z1[mt][i], g.beta);
+                    accb1[mt][i] += acc[i];
+                }
+                dZ1[mt] = acc;
+            }
+            // dL/dfeature: rows 0..7 of W0^T dZ1, i.e. registers 0..3 of each lane group (channels 4g..4g+3)
+            {
+                f32x16 dx = {};
+#pragma unroll
+                for (int ms = 0; ms < MT; ++ms)
+                {
+#pragma unroll
+                    for (int st = 0; st < S; ++st)
+                        dx = P::mma(w.w1t[ms][st][l], P::from_acc(dZ1[ms], st, [](float v, int) { return v; }), dx);
+                }
+                if (valid)
+                    *reinterpret_cast<float4*>(dfeat + (base + s) * C + 4 * hg) = make_float4(dx[0], dx[1], dx[2], dx[3]);
+            }
+            // dW(hidden) += dZ2 act(Z1)^T over the tile's samples
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+            {
+#pragma unroll
+                for (int i = 0; i < 16; ++i)
+                {
+                    const int r    = 32 * mt + crow(i, hg);
+                    sA[r * 33 + n] = dZ2[mt][i];
+                    sB[r * 33 + n] = act_f(ACT, f.z1[mt][i], g.beta);
+                }
+            }
+            wave_lds_fence();
+#pragma unroll
+            for (int st = 0; st < KSTEPS; ++st)
+            {
+#pragma unroll
+                for (int m2 = 0; m2 < MT; ++m2)
+                {
+                    auto a = stage_frag<P>(sA, 32 * m2 + n, st, hg, false);
+#pragma unroll
+                    for (int m1 = 0; m1 < MT; ++m1)
+                        dW2[m2][m1] = P::mma(a, stage_frag<P>(sB, 32 * m1 + n, st, hg, false), dW2[m2][m1]);
+                }
+            }
+            wave_lds_fence();
+            // dW(first) += dZ1 x^T over the tile's samples (N padded from 8 channels to 32)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+            {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) sA[(32 * mt + crow(i, hg)) * 33 + n] = dZ1[mt][i];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sB[(4 * hg + j) * 33 + n] = x4[j];
+            wave_lds_fence();
+#pragma unroll
+            for (int st = 0; st < KSTEPS; ++st)
+            {
+                auto b = stage_frag<P>(sB, n, st, hg, n >= C);
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) dW1[mt] = P::mma(stage_frag<P>(sA, 32 * mt + n, st, hg, false), b, dW1[mt]);
+            }
+            wave_lds_fence();
+        }
+    }
+    // ---- this wave's slice of the partial parameter gradient (packed layout) ----
+    float* part     = partials + ((int64_t)blockIdx.x * 4 + wave) * P_total;
+    const int oW0   = 0, ob0 = H * C, oW1 = ob0 + H, ob1 = oW1 + H * H, oW2 = ob1 + H, ob2 = oW2 + H;
+#pragma unroll
+    for (int m2 = 0; m2 < MT; ++m2)
+    {
+#pragma unroll
+        for (int m1 = 0; m1 < MT; ++m1)
+        {
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                part[oW1 + (32 * m2 + crow(i, hg)) * H + 32 * m1 + n] = dW2[m2][m1][i];
+        }
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+    {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (n < C) part[oW0 + (32 * mt + crow(i, hg)) * C + n] = dW1[mt][i];
+    }
+    // per-row sums over the 32 samples of a lane group: through the staging buffer
+    wave_lds_fence();
+    for (int v = 0; v < 3; ++v)
+    {
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+        {
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+            {
+                const float val = v == 0 ? accb1[mt][i] : (v == 1 ? accb2[mt][i] : accw3[mt][i]);
+                sA[(32 * mt + crow(i, hg)) * 33 + n] = val;
+            }
+        }
+        wave_lds_fence();
+        for (int r = l; r < H; r += 64)
+        {
+            float acc = 0.f;
+            for (int k = 0; k < 32; ++k) acc += sA[r * 33 + k];
+            part[(v == 0 ? ob0 : (v == 1 ? ob1 : oW2)) + r] = acc;
+        }
+        wave_lds_fence();
+    }
+    float b3 = accb3;
+#pragma unroll
+    for (int off = 16; off >= 1; off >>= 1) b3 += __shfl_xor(b3, off);
+    if (l == 0) part[ob2] = b3;
+}
+
 // ---- workspace layout ---------------------------------------------------------------------------------------------------
 struct Layout
 {
@@ -559,13 +872,20 @@ Layout proj_layout(const ProjSizes& z)
 }
 
 int bw1_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 2048)); }
+// matrix-core backward: persistent workgroups of 4 waves, one partial slice per wave
+int mfma_bwd_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 512)); }
+
+size_t partials_bytes(const ProjSizes& z)
+{
+    size_t slices = std::max<size_t>((size_t)bw1_workgroups(z), (size_t)mfma_bwd_workgroups(z) * 4);
+    return (slices * z.P * 4 + 255) / 256 * 256;
+}
 
 size_t proj_bytes(const ProjSizes& z)
 {
-    Layout L = proj_layout(z);
-    size_t partials = (size_t)bw1_workgroups(z) * z.P * 4;
-    size_t grids    = (size_t)z.max2 * z.G * z.G * z.G * z.C * 4;
-    return L.total + ((std::max(partials, (size_t)0) + 255) / 256) * 256 + grids + 256;
+    Layout L     = proj_layout(z);
+    size_t grids = (size_t)z.max2 * z.G * z.G * z.G * z.C * 4;
+    return L.total + partials_bytes(z) + grids + 256;
 }
 
 Plan plan_view(void* ws, const ProjSizes& z)
@@ -606,6 +926,9 @@ struct Args
     int act;
     const float* params;
     ProjSizes z;
+    int precision;
+    const void* grid_bf16;
+    bool mfma;  // matrix-core kernels (8 channels) rather than the vector-ALU ones
 };
 
 int make_args(Args& a, const neat_projector_args* x)
@@ -621,6 +944,9 @@ int make_args(Args& a, const neat_projector_args* x)
     NEAT_REQUIRE(K == 0 || (x->seg_node && x->seg_ray && x->seg_count && x->seg_tmin && x->seg_tmax),
                  "projector: null segment arrays");
     NEAT_REQUIRE(K < INT32_MAX && S < ((int64_t)1 << 40), "projector: too many segments");
+    NEAT_REQUIRE(x->precision == NEAT_PREC_FP32 || x->precision == NEAT_PREC_BF16 ||
+                     x->precision == NEAT_PREC_FP32_VALU,
+                 "projector: unknown precision %d", x->precision);
     const int64_t* st = x->grid_strides;
     const int64_t gs  = (int64_t)G * G * G * C;
     if (!(st[1] == 1 && st[4] == C && st[3] == (int64_t)G * C && st[2] == (int64_t)G * G * C && st[0] == gs))
@@ -633,14 +959,23 @@ int make_args(Args& a, const neat_projector_args* x)
                     "relu/silu hidden activations and a softplus output (got in=%d C=%d hidden=%d layers=%d out=%d "
                     "act=%d last=%d)",
                     x->in_features, C, x->hidden, x->hidden_layers, x->out_features, x->act, x->last_act);
+    if (x->precision == NEAT_PREC_BF16)
+    {
+        if (C != 8) return fail(NEAT_E_UNSUPPORTED, "projector: the BF16 kernels need 8 channels (got %d)", C);
+        NEAT_REQUIRE(x->grid_bf16 && (reinterpret_cast<uintptr_t>(x->grid_bf16) % 16) == 0,
+                     "projector: NEAT_PREC_BF16 needs a 16-byte aligned grid_bf16");
+    }
     NEAT_REQUIRE((reinterpret_cast<uintptr_t>(x->grid) % 16) == 0, "projector: grid must be 16-byte aligned");
     a.g = ProjGeom{x->node_pos_min, x->node_pos_max, x->node_active_prefix_sum, x->num_nodes, x->ray_origin,
                    x->ray_direction, R, x->ray_seg_offset, K, x->seg_node, x->seg_ray, x->seg_count, x->seg_tmin,
                    x->seg_tmax, x->jitter, x->seed, x->grid, N, G, x->softplus_beta};
-    a.m      = MLP{x->in_features, x->hidden, x->hidden_layers, x->out_features};
-    a.act    = x->act;
-    a.params = x->params;
-    a.z      = proj_sizes(K, S, N, G, C, a.m.nparams());
+    a.m         = MLP{x->in_features, x->hidden, x->hidden_layers, x->out_features};
+    a.act       = x->act;
+    a.params    = x->params;
+    a.z         = proj_sizes(K, S, N, G, C, a.m.nparams());
+    a.precision = x->precision;
+    a.grid_bf16 = x->grid_bf16;
+    a.mfma      = x->precision != NEAT_PREC_FP32_VALU && C == 8;
     return NEAT_OK;
 }
 
@@ -655,7 +990,7 @@ int check_ws(const Args& a, void* ws, size_t bytes)
 char* region(void* ws, const ProjSizes& z, int i) { return static_cast<char*>(ws) + proj_layout(z).off[i]; }
 
 template <int C, int H>
-int launch_fwd_ch(hipStream_t s, const Args& a, const Plan& p, float* seg_partial)
+int launch_fwd_valu(hipStream_t s, const Args& a, const Plan& p, float* seg_partial)
 {
     unsigned grid = (unsigned)std::max<int64_t>(1, a.z.max1);
     if (a.act == NEAT_ACT_RELU)
@@ -666,17 +1001,23 @@ int launch_fwd_ch(hipStream_t s, const Args& a, const Plan& p, float* seg_partia
     return NEAT_OK;
 }
 
-template <int C, int H>
-int launch_bw_ch(hipStream_t s, const Args& a, const Plan& p, const float* grad_proj, float* wt, float* dfeat,
-                 float* partials, float* grid_partials, float* grad_grid, float* grad_params)
+template <typename P, int H>
+int launch_fwd_mfma(hipStream_t s, const Args& a, const Plan& p, float* seg_partial)
 {
-    const int nwg = bw1_workgroups(a.z);
-    const int P   = a.m.nparams();
-    NEAT_HIP(hipMemsetAsync(partials, 0, (size_t)nwg * P * sizeof(float), s));
-    k_proj_bw1<C, H><<<nwg, kTile, 0, s>>>(a.g, p, a.m, wt, a.act, grad_proj, dfeat, partials);
+    unsigned grid = (unsigned)std::max<int64_t>(1, a.z.max1);
+    if (a.act == NEAT_ACT_RELU)
+        k_mfma_fwd<P, H, NEAT_ACT_RELU><<<grid, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, seg_partial);
+    else
+        k_mfma_fwd<P, H, NEAT_ACT_SILU><<<grid, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, seg_partial);
     NEAT_LAUNCHED();
-    k_dec_reduce_partials<<<grid_for(P, 256), 256, 0, s>>>(partials, nwg, P, grad_params);
-    NEAT_LAUNCHED();
+    return NEAT_OK;
+}
+
+// Scatter of dL/dfeature into the node gradient grids (pass 2), shared by both backward variants.
+template <int C>
+int launch_scatter(hipStream_t s, const Args& a, const Plan& p, const float* dfeat, float* grid_partials,
+                   float* grad_grid)
+{
     const int32_t gsize = a.g.G * a.g.G * a.g.G * C;
     const size_t smem   = (size_t)gsize * sizeof(float);
     NEAT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_proj_bw2<C>),
@@ -690,15 +1031,79 @@ int launch_bw_ch(hipStream_t s, const Args& a, const Plan& p, const float* grad_
     return NEAT_OK;
 }
 
-#define NEAT_PROJ_DISPATCH(CALL)                                   \
-    do                                                             \
-    {                                                              \
-        const int C_ = a.g.N ? (int)a.m.in : 0, H_ = a.m.hidden;   \
-        if (C_ == 8 && H_ == 64) return CALL(8, 64);               \
-        if (C_ == 8 && H_ == 32) return CALL(8, 32);               \
-        if (C_ == 4 && H_ == 64) return CALL(4, 64);               \
-        if (C_ == 4 && H_ == 32) return CALL(4, 32);               \
-    } while (0)
+template <int C, int H>
+int launch_bw_valu(hipStream_t s, const Args& a, const Plan& p, const float* grad_proj, float* wt, float* dfeat,
+                   float* partials, float* grid_partials, float* grad_grid, float* grad_params)
+{
+    const int nwg = bw1_workgroups(a.z);
+    const int P   = a.m.nparams();
+    k_transpose_params<<<grid_for(P, 256), 256, 0, s>>>(a.params, a.m, wt);
+    NEAT_LAUNCHED();
+    NEAT_HIP(hipMemsetAsync(partials, 0, (size_t)nwg * P * sizeof(float), s));
+    k_proj_bw1<C, H><<<nwg, kTile, 0, s>>>(a.g, p, a.m, wt, a.act, grad_proj, dfeat, partials);
+    NEAT_LAUNCHED();
+    k_dec_reduce_partials<<<grid_for(P, 256), 256, 0, s>>>(partials, nwg, P, grad_params);
+    NEAT_LAUNCHED();
+    return launch_scatter<C>(s, a, p, dfeat, grid_partials, grad_grid);
+}
+
+template <typename P, int H>
+int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const float* grad_proj, float* dfeat, float* partials,
+                   float* grid_partials, float* grad_grid, float* grad_params)
+{
+    const int nwg  = mfma_bwd_workgroups(a.z);
+    const int Ptot = a.m.nparams();
+    if (a.act == NEAT_ACT_RELU)
+        k_mfma_bwd<P, H, NEAT_ACT_RELU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, grad_proj, dfeat, partials,
+                                                             Ptot);
+    else
+        k_mfma_bwd<P, H, NEAT_ACT_SILU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, grad_proj, dfeat, partials,
+                                                             Ptot);
+    NEAT_LAUNCHED();
+    k_dec_reduce_partials<<<grid_for(Ptot, 256), 256, 0, s>>>(partials, nwg * 4, Ptot, grad_params);
+    NEAT_LAUNCHED();
+    return launch_scatter<8>(s, a, p, dfeat, grid_partials, grad_grid);
+}
+
+int dispatch_fwd(hipStream_t s, const Args& a, const Plan& p, float* seg_partial)
+{
+    const int C = a.m.in, H = a.m.hidden;
+    if (a.mfma)
+    {
+        if (a.precision == NEAT_PREC_BF16)
+            return H == 64 ? launch_fwd_mfma<PrecBF16, 64>(s, a, p, seg_partial)
+                           : launch_fwd_mfma<PrecBF16, 32>(s, a, p, seg_partial);
+        return H == 64 ? launch_fwd_mfma<PrecF32, 64>(s, a, p, seg_partial)
+                       : launch_fwd_mfma<PrecF32, 32>(s, a, p, seg_partial);
+    }
+    if (C == 8) return H == 64 ? launch_fwd_valu<8, 64>(s, a, p, seg_partial) : launch_fwd_valu<8, 32>(s, a, p, seg_partial);
+    return H == 64 ? launch_fwd_valu<4, 64>(s, a, p, seg_partial) : launch_fwd_valu<4, 32>(s, a, p, seg_partial);
+}
+
+int dispatch_bwd(hipStream_t s, const Args& a, const Plan& p, const float* gproj, float* wt, float* dfeat,
+                 float* partials, float* gpart, float* ggrid, float* gparams)
+{
+    const int C = a.m.in, H = a.m.hidden;
+    if (a.mfma)
+    {
+        if (a.precision == NEAT_PREC_BF16)
+            return H == 64 ? launch_bw_mfma<PrecBF16, 64>(s, a, p, gproj, dfeat, partials, gpart, ggrid, gparams)
+                           : launch_bw_mfma<PrecBF16, 32>(s, a, p, gproj, dfeat, partials, gpart, ggrid, gparams);
+        return H == 64 ? launch_bw_mfma<PrecF32, 64>(s, a, p, gproj, dfeat, partials, gpart, ggrid, gparams)
+                       : launch_bw_mfma<PrecF32, 32>(s, a, p, gproj, dfeat, partials, gpart, ggrid, gparams);
+    }
+    if (C == 8)
+        return H == 64 ? launch_bw_valu<8, 64>(s, a, p, gproj, wt, dfeat, partials, gpart, ggrid, gparams)
+                       : launch_bw_valu<8, 32>(s, a, p, gproj, wt, dfeat, partials, gpart, ggrid, gparams);
+    return H == 64 ? launch_bw_valu<4, 64>(s, a, p, gproj, wt, dfeat, partials, gpart, ggrid, gparams)
+                   : launch_bw_valu<4, 32>(s, a, p, gproj, wt, dfeat, partials, gpart, ggrid, gparams);
+}
+
+__global__ void __launch_bounds__(256) k_to_bf16(const float* src, int64_t n, unsigned short* dst)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (unsigned short)bf16_bits(src[i]);
+}
 
 }  // namespace
 
@@ -709,6 +1114,15 @@ extern "C"
         Args a;
         if (make_args(a, x) != NEAT_OK) return 0;
         return proj_bytes(a.z);
+    }
+
+    int neat_grid_to_bf16(neat_stream_t stream, const float* src, int64_t n, uint16_t* dst)
+    {
+        NEAT_REQUIRE(n >= 0 && (n == 0 || (src && dst)), "neat_grid_to_bf16: bad arguments");
+        if (n == 0) return NEAT_OK;
+        k_to_bf16<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(src, n, reinterpret_cast<unsigned short*>(dst));
+        NEAT_LAUNCHED();
+        return NEAT_OK;
     }
 
     int neat_projector_plan(neat_stream_t stream, const neat_projector_args* x, void* ws, size_t bytes)
@@ -763,7 +1177,7 @@ extern "C"
         int st = make_args(a, x);
         if (st) return st;
         if ((st = check_ws(a, ws, bytes))) return st;
-        NEAT_REQUIRE(proj || a.z.K == 0 && x->num_rays == 0, "projector: null output");
+        NEAT_REQUIRE(proj || (a.z.K == 0 && x->num_rays == 0), "projector: null output");
         hipStream_t s      = as_stream(stream);
         Plan p             = plan_view(ws, a.z);
         float* seg_partial = reinterpret_cast<float*>(region(ws, a.z, 11));
@@ -772,14 +1186,7 @@ extern "C"
             // segments whose node has no valid slot are in no chunk: they contribute zero
             k_fill_f32<<<grid_for(a.z.K, 256), 256, 0, s>>>(seg_partial, a.z.K, 0.f);
             NEAT_LAUNCHED();
-#define FWD_CALL(C, H) launch_fwd_ch<C, H>(s, a, p, seg_partial)
-            st = NEAT_OK;
-            [&]() -> int {
-                NEAT_PROJ_DISPATCH(st = FWD_CALL);
-                return 0;
-            }();
-#undef FWD_CALL
-            if (st) return st;
+            if ((st = dispatch_fwd(s, a, p, seg_partial))) return st;
         }
         if (x->num_rays > 0)
         {
@@ -798,33 +1205,23 @@ extern "C"
         if (st) return st;
         if ((st = check_ws(a, ws, bytes))) return st;
         NEAT_REQUIRE(grad_grid && grad_params && (grad_proj || x->num_rays == 0), "projector: null gradient buffer");
-        hipStream_t s       = as_stream(stream);
-        const auto& z       = a.z;
-        Plan p              = plan_view(ws, z);
-        float* wt           = reinterpret_cast<float*>(region(ws, z, 12));
-        float* dfeat        = reinterpret_cast<float*>(region(ws, z, 13));
-        const Layout L      = proj_layout(z);
-        float* partials     = reinterpret_cast<float*>(static_cast<char*>(ws) + L.total);
-        const size_t pbytes = ((size_t)bw1_workgroups(z) * z.P * 4 + 255) / 256 * 256;
-        float* gpartials    = reinterpret_cast<float*>(static_cast<char*>(ws) + L.total + pbytes);
-        k_transpose_params<<<grid_for(z.P, 256), 256, 0, s>>>(a.params, a.m, wt);
-        NEAT_LAUNCHED();
+        hipStream_t s    = as_stream(stream);
+        const auto& z    = a.z;
+        Plan p           = plan_view(ws, z);
+        float* wt        = reinterpret_cast<float*>(region(ws, z, 12));
+        float* dfeat     = reinterpret_cast<float*>(region(ws, z, 13));
+        const Layout L   = proj_layout(z);
+        float* partials  = reinterpret_cast<float*>(static_cast<char*>(ws) + L.total);
+        float* gpartials = reinterpret_cast<float*>(static_cast<char*>(ws) + L.total + partials_bytes(z));
         if (z.K == 0)
         {
-            k_fill_f32<<<grid_for(z.N * z.G * z.G * z.G * z.C, 256), 256, 0, s>>>(grad_grid,
-                                                                               z.N * z.G * z.G * z.G * z.C, 0.f);
+            const int64_t ng = z.N * z.G * z.G * z.G * z.C;
+            k_fill_f32<<<grid_for(ng, 256), 256, 0, s>>>(grad_grid, ng, 0.f);
             NEAT_LAUNCHED();
             k_fill_f32<<<grid_for(z.P, 256), 256, 0, s>>>(grad_params, z.P, 0.f);
             NEAT_LAUNCHED();
             return NEAT_OK;
         }
-#define BW_CALL(C, H) launch_bw_ch<C, H>(s, a, p, grad_proj, wt, dfeat, partials, gpartials, grad_grid, grad_params)
-        st = NEAT_OK;
-        [&]() -> int {
-            NEAT_PROJ_DISPATCH(st = BW_CALL);
-            return 0;
-        }();
-#undef BW_CALL
-        return st;
+        return dispatch_bwd(s, a, p, grad_proj, wt, dfeat, partials, gpartials, grad_grid, grad_params);
     }
 }

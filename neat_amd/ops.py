@@ -159,13 +159,22 @@ def mse_fp64(pred, target, grad_scale=1.0, want_grad=True, stream=None):
     return loss, grad
 
 
+def grid_to_bf16(grid, stream=None):
+    """Round-to-nearest-even BF16 bit patterns (int16) of an FP32 grid, same shape and strides (neat_grid_to_bf16)."""
+    if not grid.is_contiguous(memory_format=torch.channels_last_3d) and not grid.is_contiguous():
+        raise ValueError("grid_to_bf16: grid must be dense")
+    out = torch.empty_like(grid, dtype=torch.int16)
+    call("neat_grid_to_bf16", stream_ptr(stream), ptr(grid), grid.numel(), ptr(out))
+    return out
+
+
 class Projector:
     """Fused projector over one batch's canonical segment list (neat_projector_plan / _fwd / _bwd).
 
     Holds references to every tensor whose address is in the argument struct, and the workspace the plan fills."""
 
     def __init__(self, pos_min, pos_max, prefix, origin, direction, segs, grid, params, mlp, jitter=False, seed=0,
-                 stream=None):
+                 stream=None, precision="fp32", grid_bf16=None):
         self.stream = stream
         N, C, G = grid.shape[0], grid.shape[1], grid.shape[2]
         R = origin.shape[0]
@@ -189,6 +198,16 @@ class Projector:
         a.in_features, a.hidden, a.hidden_layers, a.out_features = mlp["in"], mlp["hidden"], mlp["layers"], mlp["out"]
         a.params, a.act, a.last_act, a.softplus_beta = params.data_ptr(), _abi.ACT[mlp["act"]], \
             _abi.ACT[mlp["last_act"]], mlp["beta"]
+        a.precision = _abi.PREC[precision]
+        if precision == "bf16":
+            if grid_bf16 is None:
+                grid_bf16 = grid_to_bf16(grid, stream)
+            if grid_bf16.dtype != torch.int16 or grid_bf16.stride() != grid.stride() or \
+                    grid_bf16.shape != grid.shape:
+                raise ValueError("grid_bf16 must be an int16 tensor with the grid's shape and strides")
+            self._keep.append(grid_bf16)
+            a.grid_bf16 = grid_bf16.data_ptr()
+        self.grid_bf16 = grid_bf16
         self.args = a
         self.grid, self.params, self.R, self.S = grid, params, R, S
         nbytes = _abi.lib().neat_projector_workspace_bytes(ctypes.byref(a))
