@@ -233,6 +233,10 @@ typedef struct neat_projector_args
     int32_t precision;
     /* BF16 copy of the grid, same shape and channels-last layout (NEAT_PREC_BF16 only; see neat_grid_to_bf16) */
     const uint16_t* grid_bf16;
+    /* optional proposal CDF [num_segments][8] (see neat_proposal_fwd): a segment whose row has cdf[7] > 0 places its
+     * seg_count samples in equal-probability cells of that piecewise-constant density, each weighted by its cell's
+     * length; rows with cdf[7] == 0, or a NULL pointer, keep the uniform samples of the reference */
+    const float* seg_cdf;
 } neat_projector_args;
 
 #define NEAT_PREC_FP32 0
@@ -258,6 +262,46 @@ int neat_projector_fwd(neat_stream_t stream, const neat_projector_args* args, in
  * (packed, written). Samples are recomputed; no activations are kept from the forward. */
 int neat_projector_bwd(neat_stream_t stream, const neat_projector_args* args, const float* grad_proj,
                        float* grad_grid, float* grad_params, void* workspace, size_t workspace_bytes);
+
+/* Per-sample decoded values softplus(decoder(trilinear(grid))) (unweighted) of the planned segment list, in the
+ * plan's node-major sample order (values [num_samples]), and seg_value_offset [num_segments] = the first sample of each
+ * canonical segment in that order. Matrix-core kernels only (8 channels, NEAT_PREC_FP32 or NEAT_PREC_BF16). */
+int neat_projector_sample_values(neat_stream_t stream, const neat_projector_args* args, float* values,
+                                 int64_t* seg_value_offset, void* workspace, size_t workspace_bytes);
+
+/* ---- learned proposal (no reference counterpart) -----------------------------------------------------------------
+ * A per-segment MLP 6 -> 32 -> 32 -> 8 (BF16 on the matrix cores, FP32 accumulation) from the segment's entry and exit
+ * points o + d tmin, o + d tmax to a piecewise-constant density over 8 equal bins of [tmin, tmax]:
+ * p = 0.75 softmax(logits) + 0.25 / 8. Packed parameters W0 [32,6], b0 [32], W1 [32,32], b1 [32], W2 [8,32], b2 [8]. */
+int64_t neat_proposal_num_params(void);
+
+/* seg_cdf [num_segments][8] = cumulative p (cdf[7] = 1) for segments with seg_count > num_samples, whose count becomes
+ * num_samples in seg_count_out (may be NULL); other segments keep their count and get a zero row (uniform sampling,
+ * the fallback). logits [num_segments][8] (may be NULL) receives the raw logits. */
+int neat_proposal_fwd(neat_stream_t stream, const float* ray_origin, const float* ray_direction,
+                      const int32_t* seg_ray, const float* seg_tmin, const float* seg_tmax, const int32_t* seg_count,
+                      int64_t num_segments, const float* params, int32_t num_samples, float* seg_cdf,
+                      int32_t* seg_count_out, float* logits);
+
+/* Training target [num_segments][8]: bin b gets the total variation of the field's values between consecutive uniform
+ * samples whose midpoint lies in bin b, plus 5 % of the mean, normalised (values[seg_value_offset[k] + j], j <
+ * seg_count[k]; see neat_projector_sample_values). */
+int neat_proposal_targets(neat_stream_t stream, const float* values, const int64_t* seg_value_offset,
+                          const int32_t* seg_count, int64_t num_segments, float* target);
+
+/* grad_params = gradient of scale * sum_k CE(target_k, softmax(logits_k)) (scale = 1 / num_segments for the mean). */
+size_t neat_proposal_workspace_bytes(int64_t num_segments);
+int neat_proposal_bwd(neat_stream_t stream, const float* ray_origin, const float* ray_direction, const int32_t* seg_ray,
+                      const float* seg_tmin, const float* seg_tmax, int64_t num_segments, const float* params,
+                      const float* target, float scale, float* grad_params, void* workspace, size_t workspace_bytes);
+
+/* ---- optimizer --------------------------------------------------------------------------------------------------------
+ * One Adam step over n FP32 parameters in place (replaces torch::optim::Adam::step for the grid and decoder parameter
+ * groups of HierarchicalNeuralGeometry::OptimizerStep, geometry.h:66-88; groups geometry_ex_ex.cpp:40-49,
+ * geometry.cpp:90-97). step counts from 1; weight_decay adds weight_decay * param to the gradient as torch does.
+ * All four buffers 16-byte aligned. */
+int neat_adam_step(neat_stream_t stream, float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
+                   int64_t n, double lr, double beta1, double beta2, double eps, double weight_decay, int64_t step);
 
 #ifdef __cplusplus
 }

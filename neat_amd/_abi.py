@@ -63,7 +63,7 @@ class ProjectorArgs(ctypes.Structure):
         ("grid", _P), ("num_slots", _I64), ("channels", _I32), ("grid_size", _I32), ("grid_strides", _I64 * 5),
         ("in_features", _I32), ("hidden", _I32), ("hidden_layers", _I32), ("out_features", _I32),
         ("params", _P), ("act", _I32), ("last_act", _I32), ("softplus_beta", _F),
-        ("precision", _I32), ("grid_bf16", _P),
+        ("precision", _I32), ("grid_bf16", _P), ("seg_cdf", _P),
     ]
 
 
@@ -77,6 +77,14 @@ SIGNATURES.update({
     "neat_projector_fwd": (ctypes.c_int, [_P, _PA, _I32, _P, _P, _SZ]),
     "neat_projector_bwd": (ctypes.c_int, [_P, _PA, _P, _P, _P, _P, _SZ]),
     "neat_grid_to_bf16": (ctypes.c_int, [_P, _P, _I64, _P]),
+    "neat_projector_sample_values": (ctypes.c_int, [_P, _PA, _P, _P, _P, _SZ]),
+    "neat_proposal_num_params": (_I64, []),
+    "neat_proposal_fwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _I64, _P, _I32, _P, _P, _P]),
+    "neat_proposal_targets": (ctypes.c_int, [_P, _P, _P, _P, _I64, _P]),
+    "neat_proposal_workspace_bytes": (_SZ, [_I64]),
+    "neat_proposal_bwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I64, _P, _P, _F, _P, _P, _SZ]),
+    "neat_adam_step": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, _I64]),
 })
 
 _LIB = None

@@ -124,7 +124,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("softplus_beta", &Netparams::softplus_beta)
         .def_readwrite("decoder_activation", &Netparams::decoder_activation)
         .def_readwrite("decoder_hidden_layers", &Netparams::decoder_hidden_layers)
-        .def_readwrite("decoder_hidden_features", &Netparams::decoder_hidden_features);
+        .def_readwrite("decoder_hidden_features", &Netparams::decoder_hidden_features)
+        .def_readwrite("decoder_lr", &Netparams::decoder_lr);
     py::class_<TreeOptimizerParams>(m, "TreeOptimizerParams")
         .def(py::init<>())
         .def_readwrite("max_active_nodes", &TreeOptimizerParams::max_active_nodes);
@@ -136,6 +137,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     py::class_<MyTrainParams>(m, "MyTrainParams")
         .def(py::init<>())
         .def_readwrite("grid_init", &MyTrainParams::grid_init)
+        .def_readwrite("lr_exex_grid_adam", &MyTrainParams::lr_exex_grid_adam)
         .def_readwrite("use_fused_projector", &MyTrainParams::use_fused_projector)
         .def_readwrite("compensated_sum", &MyTrainParams::compensated_sum)
         .def_readwrite("projector_precision", &MyTrainParams::projector_precision);
@@ -155,6 +157,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("DecodeFeatures", &GeometryExEx::DecodeFeatures)
         .def("IntegrateSamplesXRay", &GeometryExEx::IntegrateSamplesXRay)
         .def("FusedUnsupportedReason", &GeometryExEx::FusedUnsupportedReason)
+        .def("CreateGeometryOptimizer", &GeometryExEx::CreateGeometryOptimizer)
+        .def("OptimizerStep", &GeometryExEx::OptimizerStep)
+        .def("UpdateLearningRate", &GeometryExEx::UpdateLearningRate)
         .def_property_readonly("decoder", [](GeometryExEx& g) { return g.decoder.ptr(); })
         .def_property_readonly("grid", [](GeometryExEx& g) { return g.explicit_grid_generator->grid_data; });
 }

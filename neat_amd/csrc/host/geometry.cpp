@@ -252,3 +252,38 @@ torch::Tensor GeometryExEx::ComputeImageFused(const SampleList& samples, int num
     return ProjectorFn::apply(explicit_grid_generator->grid_data, decoder->packed_parameters(), geo, iv,
                               (double)np.softplus_beta, params->train_params.compensated_sum);
 }
+
+void HierarchicalNeuralGeometry::CreateGeometryOptimizer()
+{
+    optimizer_adam    = std::make_shared<NeatAdam>();
+    optimizer_decoder = std::make_shared<NeatAdam>();
+    AddParametersToOptimizer();
+}
+
+void HierarchicalNeuralGeometry::AddParametersToOptimizer()
+{
+    if (decoder) optimizer_decoder->add_param_group(decoder->parameters(), params->net_params.decoder_lr);
+}
+
+void HierarchicalNeuralGeometry::OptimizerStep(int epoch_id)
+{
+    (void)epoch_id;
+    for (auto& o : {optimizer_adam, optimizer_decoder})
+    {
+        if (!o) continue;
+        o->step();
+        o->zero_grad();
+    }
+}
+
+void HierarchicalNeuralGeometry::UpdateLearningRate(double factor)
+{
+    if (optimizer_adam) optimizer_adam->update_lr(factor);
+    if (optimizer_decoder) optimizer_decoder->update_lr(factor);
+}
+
+void GeometryExEx::AddParametersToOptimizer()
+{
+    HierarchicalNeuralGeometry::AddParametersToOptimizer();
+    optimizer_adam->add_param_group(explicit_grid_generator->parameters(), params->train_params.lr_exex_grid_adam);
+}

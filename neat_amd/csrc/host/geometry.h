@@ -11,6 +11,7 @@
 #include "ImplicitNet.h"
 #include "IndirectGridSample3D.h"
 #include "Settings.h"
+#include "optim.h"
 
 // out[c, ray_index[i]] += sample_values[i, c] * integration_weight[i]; deterministic when ray_sample_offset
 // (CSR offsets of ray-sorted samples) is given.
@@ -52,8 +53,18 @@ class HierarchicalNeuralGeometry : public NeuralGeometry
     // The fused projector for a SampleList with segments, or an undefined tensor when not covered.
     virtual torch::Tensor ComputeImageFused(const SampleList& samples, int num_pixels) { return {}; }
 
+    // Adam groups as the reference creates them (geometry.h:46-56, geometry.cpp:87-97): the decoder in its own
+    // optimizer at net_params.decoder_lr; subclasses add their parameters to optimizer_adam.
+    void CreateGeometryOptimizer();
+    void ResetGeometryOptimizer() { CreateGeometryOptimizer(); }
+    virtual void AddParametersToOptimizer();
+    // step and zero the gradients of every optimizer (geometry.h:66-88)
+    void OptimizerStep(int epoch_id);
+    void UpdateLearningRate(double factor);
+
     HyperTreeBase tree = nullptr;
     FCBlock decoder    = nullptr;
+    std::shared_ptr<NeatAdam> optimizer_adam, optimizer_decoder;
 };
 
 class GeometryExEx : public HierarchicalNeuralGeometry
@@ -66,6 +77,9 @@ class GeometryExEx : public HierarchicalNeuralGeometry
 
     // why the fused projector does not cover this configuration ("" when it does)
     std::string FusedUnsupportedReason();
+
+    // the grid group at train_params.lr_exex_grid_adam (geometry_ex_ex.cpp:40-49)
+    void AddParametersToOptimizer() override;
 
     ExplicitFeatureGrid explicit_grid_generator = nullptr;
 };

@@ -256,4 +256,31 @@ __device__ __forceinline__ void tile_forward_mfma(const WeightLDS<P, H>& w, cons
     f.z = part + __shfl_xor(part, 32) + w.vec[3][0];
 }
 
+// ---- weight-gradient products over a tile's samples -----------------------------------------------------------
+__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+constexpr int kRS = 36;  // row stride (floats) of the backward's [row][sample] staging buffers (16-byte rows)
+
+// A / B fragment of a weight-gradient product over the tile's 32 samples, from a [row][sample] staging buffer:
+// FP32 step st covers samples 2st + l/32; BF16 step st covers samples 16st + 8(l/32) + j.
+template <typename P>
+__device__ __forceinline__ typename P::Frag stage_frag(const float* buf, int row, int st, int hg, bool zero)
+{
+    typename P::Frag f;
+    if constexpr (P::kBF16)
+    {
+        // masked lanes read row 0 (always inside the buffer) and discard it
+        const float4* src = reinterpret_cast<const float4*>(buf + (zero ? 0 : row) * kRS + 16 * st + 8 * hg);
+        const float4 u = src[0], v = src[1];
+        const float e[8] = {u.x, u.y, u.z, u.w, v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) f[j] = (__bf16)(zero ? 0.f : e[j]);
+    }
+    else
+    {
+        f = zero ? 0.f : buf[row * kRS + 2 * st + hg];
+    }
+    return f;
+}
+
 }  // namespace neat

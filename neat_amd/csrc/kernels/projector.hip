@@ -23,7 +23,9 @@
 namespace neat
 {
 
-constexpr int kFwdChunk   = 4096;  // samples per forward / backward-pass-1 chunk
+constexpr int kFwdChunk   = 4096;  // samples per forward / backward-pass-1 chunk (and per forward window)
+constexpr int kSegCost    = 16;    // chunk cost of a segment, in samples
+constexpr int kSegCap     = kFwdChunk / kSegCost;  // segments per forward chunk, at most
 constexpr int kFwdThreads = 256;
 
 struct ProjGeom
@@ -48,6 +50,7 @@ struct ProjGeom
     int64_t N;
     int32_t G;
     float beta;
+    const float* seg_cdf;  // optional [K][8] proposal CDF (row used when cdf[7] > 0)
 };
 
 struct Plan
@@ -75,6 +78,64 @@ struct Sample
     int64_t ray;
 };
 
+// Proposal-guided placement: n equal-probability cells of the piecewise-constant density over 8 equal bins whose CDF
+// is cdf (cell ends pinned to tmin / tmax); the sample sits at t1 (1 - a) + t2 a in its cell, weighted by the cell's
+// length. Same operations as the oracle's proposal_cell / sample_tw.
+__device__ __forceinline__ float inv_cdf8(const float (&cdf)[8], float u)
+{
+    int b    = 0;
+    float lo = 0.f, hi = cdf[0];
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+    {
+        if (b == i && cdf[i] <= u)
+        {
+            lo = cdf[i];
+            hi = cdf[i + 1];
+            b  = i + 1;
+        }
+    }
+    const float pb = hi - lo;
+    float f        = pb > 0.f ? (u - lo) / pb : 0.f;
+    f              = fminf(fmaxf(f, 0.f), 1.f);
+    return ((float)b + f) * 0.125f;
+}
+
+__device__ __forceinline__ float proposal_cell(const float (&cdf)[8], float tmin, float tmax, int32_t n, int32_t j)
+{
+    if (j <= 0) return tmin;
+    if (j >= n) return tmax;
+    return tmin + (tmax - tmin) * inv_cdf8(cdf, (float)j / (float)n);
+}
+
+// t and w of sample j of a segment: uniform (HyperTree.cu:339-389) or inside the proposal's cell j.
+__device__ __forceinline__ void sample_tw(const ProjGeom& g, bool prop, const float (&cdf)[8], float tmin, float tmax,
+                                          float step, int32_t n, int32_t j, int64_t ray, int32_t rank, float& t,
+                                          float& w)
+{
+    const float a = seg_a(g.jitter, g.seed, ray, rank, j);
+    if (!prop)
+    {
+        t = sample_t(tmin, step, j, a);
+        w = sample_w(tmin, tmax, step, n, j, t, g.jitter, g.seed, ray, rank);
+        return;
+    }
+    const float t1 = proposal_cell(cdf, tmin, tmax, n, j);
+    const float t2 = proposal_cell(cdf, tmin, tmax, n, j + 1);
+    t              = t1 * (1.0f - a) + t2 * a;
+    w              = t2 - t1;
+}
+
+__device__ __forceinline__ bool load_cdf(const ProjGeom& g, int64_t k, float (&cdf)[8])
+{
+    if (!g.seg_cdf) return false;
+    const float4* r = reinterpret_cast<const float4*>(g.seg_cdf + 8 * k);
+    const float4 u = r[0], v = r[1];
+    cdf[0] = u.x; cdf[1] = u.y; cdf[2] = u.z; cdf[3] = u.w;
+    cdf[4] = v.x; cdf[5] = v.y; cdf[6] = v.z; cdf[7] = v.w;
+    return cdf[7] > 0.f;
+}
+
 __device__ __forceinline__ Sample sample_geo(const ProjGeom& g, int32_t k, int32_t j)
 {
     Sample s;
@@ -84,8 +145,10 @@ __device__ __forceinline__ Sample sample_geo(const ProjGeom& g, int32_t k, int32
     const int32_t node = g.seg_node[k];
     const float tmin = g.seg_tmin[k], tmax = g.seg_tmax[k];
     const float step = (tmax - tmin) / (float)n;
-    const float t    = sample_t(tmin, step, j, seg_a(g.jitter, g.seed, ray, rank, j));
-    s.w              = sample_w(tmin, tmax, step, n, j, t, g.jitter, g.seed, ray, rank);
+    float cdf[8];
+    const bool prop = load_cdf(g, k, cdf);
+    float t;
+    sample_tw(g, prop, cdf, tmin, tmax, step, n, j, ray, rank, t, s.w);
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
@@ -243,12 +306,16 @@ __global__ void __launch_bounds__(256) k_plan_slots(ProjGeom g, const int32_t* s
     p.slot_begin[s] = (int32_t)lo;
 }
 
+// Forward / backward-pass-1 chunks are cut by the cost samples + kSegCost x segments, in steps of kFwdChunk: a
+// chunk then holds fewer than kFwdChunk / kSegCost + 1 segments and fewer than kFwdChunk samples plus its last
+// segment's.
 __global__ void __launch_bounds__(256) k_plan_chunk_counts(ProjGeom g, Plan p)
 {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= g.N) return;
-    int64_t n       = p.nm_soff[p.slot_begin[s + 1]] - p.nm_soff[p.slot_begin[s]];
-    p.chunk_off1[s] = (n + kFwdChunk - 1) / kFwdChunk;
+    const int32_t sb = p.slot_begin[s], se = p.slot_begin[s + 1];
+    int64_t n       = p.nm_soff[se] - p.nm_soff[sb];
+    p.chunk_off1[s] = (n + (int64_t)kSegCost * (se - sb) + kFwdChunk - 1) / kFwdChunk;
     p.chunk_off2[s] = (n + p.chunk2 - 1) / p.chunk2;
 }
 
@@ -266,8 +333,24 @@ __device__ __forceinline__ int32_t lower_seg(const int64_t* nm_soff, int32_t lo,
     return lo;
 }
 
-// One thread per chunk: chunk r of a slot covers the slot's segments whose first sample lies in [r*T, (r+1)*T)
-// (relative to the slot's first sample). Threads [0, max1) fill forward chunks, [max1, max1 + max2) scatter chunks.
+// First node-major segment q in [lo, hi) with cost(q) = nm_soff[q] - base + kSegCost (q - sb) >= target.
+__device__ __forceinline__ int32_t lower_cost(const int64_t* nm_soff, int32_t sb, int32_t lo, int32_t hi, int64_t base,
+                                              int64_t target)
+{
+    while (lo < hi)
+    {
+        int32_t mid = (lo + hi) >> 1;
+        if (nm_soff[mid] - base + (int64_t)kSegCost * (mid - sb) < target)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// One thread per chunk. Scatter chunk r of a slot covers the slot's segments whose first sample lies in
+// [r*T, (r+1)*T) (relative to the slot's first sample); forward chunk r those whose cost lies in that range.
+// Threads [0, max1) fill forward chunks, [max1, max1 + max2) scatter chunks.
 __global__ void __launch_bounds__(256) k_plan_chunks(ProjGeom g, Plan p)
 {
     int64_t i       = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -290,8 +373,17 @@ __global__ void __launch_bounds__(256) k_plan_chunks(ProjGeom g, Plan p)
     const int64_t T    = pass1 ? kFwdChunk : p.chunk2;
     const int32_t sb   = p.slot_begin[s], se = p.slot_begin[s + 1];
     const int64_t base = p.nm_soff[sb];
-    const int32_t q0   = lower_seg(p.nm_soff, sb, se, base + r * T);
-    const int32_t q1   = (i + 1 < off[s + 1]) ? lower_seg(p.nm_soff, sb, se, base + (r + 1) * T) : se;
+    int32_t q0, q1;
+    if (pass1)
+    {
+        q0 = lower_cost(p.nm_soff, sb, sb, se, base, r * T);
+        q1 = (i + 1 < off[s + 1]) ? lower_cost(p.nm_soff, sb, sb, se, base, (r + 1) * T) : se;
+    }
+    else
+    {
+        q0 = lower_seg(p.nm_soff, sb, se, base + r * T);
+        q1 = (i + 1 < off[s + 1]) ? lower_seg(p.nm_soff, sb, se, base + (r + 1) * T) : se;
+    }
     (pass1 ? p.chunk1_slot : p.chunk2_slot)[i] = (int32_t)s;
     (pass1 ? p.chunk1_q0 : p.chunk2_q0)[i]     = q0;
     (pass1 ? p.chunk1_q1 : p.chunk2_q1)[i]     = q1;
@@ -427,52 +519,126 @@ __global__ void __launch_bounds__(kTile) k_proj_bw1(ProjGeom g, Plan p, MLP m, c
 }
 
 // ---- backward pass 2: scatter into the node's gradient grid in LDS ----------------------------------------------------
+// Per-segment state of a sample stream: what the samples of one segment share.
+struct SegCtx
+{
+    int64_t ray;
+    int32_t rank, n;
+    float tmin, tmax, step;
+    float o[3], d[3];
+    bool prop;
+    float cdf[8];
+};
+
+__device__ __forceinline__ SegCtx seg_ctx(const ProjGeom& g, int32_t k)
+{
+    SegCtx c;
+    c.ray  = g.seg_ray[k];
+    c.n    = g.seg_count[k];
+    c.rank = (int32_t)(k - g.ray_seg_offset[c.ray]);
+    c.tmin = g.seg_tmin[k];
+    c.tmax = g.seg_tmax[k];
+    c.step = (c.tmax - c.tmin) / (float)c.n;
+    c.prop = load_cdf(g, k, c.cdf);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        c.o[d] = g.ro[3 * c.ray + d];
+        c.d[d] = g.rd[3 * c.ray + d];
+    }
+    return c;
+}
+
+// Local coordinates of sample j: the same operations as sample_geo, so the same bits.
+__device__ __forceinline__ void seg_uv(const ProjGeom& g, const SegCtx& c, const float (&mn)[3], const float (&mx)[3],
+                                       int32_t j, float (&uv)[3])
+{
+    float t;
+    if (!c.prop)
+    {
+        t = sample_t(c.tmin, c.step, j, seg_a(g.jitter, g.seed, c.ray, c.rank, j));
+    }
+    else
+    {
+        const float a  = seg_a(g.jitter, g.seed, c.ray, c.rank, j);
+        const float t1 = proposal_cell(c.cdf, c.tmin, c.tmax, c.n, j);
+        const float t2 = proposal_cell(c.cdf, c.tmin, c.tmax, c.n, j + 1);
+        t              = t1 * (1.0f - a) + t2 * a;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        float pd = c.o[d] + c.d[d] * t;
+        uv[d]    = (pd - mn[d]) / (mx[d] - mn[d]) * 2 - 1;
+    }
+}
+
+constexpr int kScatterThreads = 1024;
+
+// One workgroup per scatter chunk (a run of node-major samples of one slot). Lane l handles channel l % C of a
+// sample, so one LDS add instruction of a wave covers 64 / C samples x C channels (distinct banks for distinct
+// vertices). The chunk's samples are cut into 16 x 64 / C contiguous streams, one per lane group, so the samples
+// of one instruction lie far apart along the node-major order (different segments / cells: few same-address
+// collisions); a stream walks its samples in order and reloads the segment state only at segment boundaries.
 template <int C>
-__global__ void __launch_bounds__(256) k_proj_bw2(ProjGeom g, Plan p, const float* dfeat, float* grad_grid,
-                                                  float* partial_grids)
+__global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p, const float* dfeat,
+                                                              float* grad_grid, float* partial_grids)
 {
     extern __shared__ float sg[];
+    constexpr int kSPW = 64 / C;
     const int64_t c = blockIdx.x;
     if (c >= p.chunk_off2[g.N]) return;
-    const int32_t slot   = p.chunk2_slot[c];
-    const int32_t q0     = p.chunk2_q0[c], q1 = p.chunk2_q1[c];
-    const int32_t gsize  = g.G * g.G * g.G * C;
-    for (int32_t i = threadIdx.x; i < gsize; i += blockDim.x) sg[i] = 0.f;
+    const int32_t slot  = p.chunk2_slot[c];
+    const int32_t q0    = p.chunk2_q0[c], q1 = p.chunk2_q1[c];
+    const int32_t gsize = g.G * g.G * g.G * C;
+    for (int32_t i = threadIdx.x * 4; i < gsize; i += blockDim.x * 4)
+        *reinterpret_cast<float4*>(sg + i) = make_float4(0.f, 0.f, 0.f, 0.f);
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
-    for (int32_t q = q0 + wave; q < q1; q += nwave)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ch       = lane % C;
+    const int64_t U    = (int64_t)(blockDim.x >> 6) * kSPW;
+    const int64_t u    = (int64_t)wave * kSPW + lane / C;
+    const int64_t base = p.nm_soff[q0];
+    const int64_t ns   = p.nm_soff[q1] - base;
+    const int64_t L    = (ns + U - 1) / U;
+    int64_t s          = u * L;
+    const int64_t send = min(ns, s + L);
+    if (s < send)
     {
-        const int32_t k     = p.perm[q];
-        const int32_t n     = g.seg_count[k];
-        const int64_t sbase = p.nm_soff[q];
-        for (int32_t j = lane; j < n; j += 64)
+        const int32_t node = g.seg_node[p.perm[q0]];
+        float mn[3], mx[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
         {
-            Sample smp       = sample_geo(g, k, j);
-            Corners cr       = corners_cl(smp.uv, g.G, C);
-            const float* df  = dfeat + (sbase + j) * C;
-            float dv[C];
-#pragma unroll
-            for (int q2 = 0; q2 < C; q2 += 4)
+            mn[d] = g.pmin[3 * node + d];
+            mx[d] = g.pmax[3 * node + d];
+        }
+        int32_t q    = find_segment(p.nm_soff, q0, q1, base + s);
+        int64_t qbeg = p.nm_soff[q] - base;
+        int64_t qend = p.nm_soff[q + 1] - base;
+        SegCtx sc    = seg_ctx(g, p.perm[q]);
+        for (; s < send; ++s)
+        {
+            while (s >= qend)
             {
-                float4 v    = *reinterpret_cast<const float4*>(df + q2);
-                dv[q2]      = v.x;
-                dv[q2 + 1]  = v.y;
-                dv[q2 + 2]  = v.z;
-                dv[q2 + 3]  = v.w;
+                ++q;
+                qbeg = qend;
+                qend = p.nm_soff[q + 1] - base;
+                if (s < qend) sc = seg_ctx(g, p.perm[q]);
             }
+            float uv[3];
+            seg_uv(g, sc, mn, mx, (int32_t)(s - qbeg), uv);
+            const Corners cr = corners_cl(uv, g.G, C);
+            const float dv   = dfeat[(base + s) * C + ch];
 #pragma unroll
-            for (int kk = 0; kk < 8; ++kk)
-            {
-#pragma unroll
-                for (int q2 = 0; q2 < C; ++q2) atomicAdd(sg + cr.off[kk] + q2, cr.w[kk] * dv[q2]);
-            }
+            for (int kk = 0; kk < 8; ++kk) atomicAdd(sg + cr.off[kk] + ch, cr.w[kk] * dv);
         }
     }
     __syncthreads();
     const bool split = p.chunk_off2[slot + 1] - p.chunk_off2[slot] > 1;
     float* dst       = split ? partial_grids + c * gsize : grad_grid + (int64_t)slot * gsize;
     for (int32_t i = threadIdx.x * 4; i < gsize; i += blockDim.x * 4)
-        *reinterpret_cast<float4*>(dst + i) = make_float4(sg[i], sg[i + 1], sg[i + 2], sg[i + 3]);
+        *reinterpret_cast<float4*>(dst + i) = *reinterpret_cast<const float4*>(sg + i);
 }
 
 // Slots cut into several scatter chunks: sum the chunk grids in chunk order. Slots without samples: zero.
@@ -534,84 +700,176 @@ __device__ __forceinline__ const void* node_grid_ptr(const ProjGeom& g, const vo
                 : static_cast<const void*>(g.grid + off);
 }
 
-// Forward: per chunk, tiles of 32 samples per wave on the matrix cores, per-segment sums as in k_proj_fwd.
+// Segment state of one forward chunk, staged in LDS once per chunk: what the chunk's samples need from the segment
+// list and the rays, so that a sample's geometry costs LDS reads rather than chains of dependent global loads.
+struct ChunkCtx
+{
+    int32_t beg[kSegCap + 1];  // chunk-relative first sample of each segment; beg[nseg] = the chunk's sample count
+    int32_t k[kSegCap];        // canonical segment index
+    int32_t ray[kSegCap];
+    int32_t rank[kSegCap];
+    int32_t n[kSegCap];
+    float tmin[kSegCap], tmax[kSegCap], step[kSegCap];
+    float gproj[kSegCap];  // dL/dprojection of the segment's ray (backward)
+    float o[3][kSegCap], d[3][kSegCap];
+    float cdf[8][kSegCap];  // proposal CDF rows (cdf[7][t] == 0: uniform)
+    uint16_t sq[kFwdChunk];  // window-relative sample -> chunk-relative segment
+};
+
+__device__ void load_chunk_ctx(const ProjGeom& g, const Plan& p, int32_t q0, int32_t q1, int64_t base,
+                               const float* grad_proj, ChunkCtx& c)
+{
+    const int32_t nseg = q1 - q0;
+    for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x)
+    {
+        const int32_t q   = q0 + t;
+        const int32_t k   = p.perm[q];
+        const int32_t ray = g.seg_ray[k];
+        const int32_t n   = g.seg_count[k];
+        const float tmin = g.seg_tmin[k], tmax = g.seg_tmax[k];
+        c.beg[t] = (int32_t)(p.nm_soff[q] - base);
+        if (t == nseg - 1) c.beg[nseg] = (int32_t)(p.nm_soff[q1] - base);
+        c.k[t]     = k;
+        c.ray[t]   = ray;
+        c.rank[t]  = (int32_t)(k - g.ray_seg_offset[ray]);
+        c.n[t]     = n;
+        c.tmin[t]  = tmin;
+        c.tmax[t]  = tmax;
+        c.step[t]  = (tmax - tmin) / (float)n;
+        c.gproj[t] = grad_proj ? grad_proj[ray] : 0.f;
+        float cdf[8];
+        const bool prop = load_cdf(g, k, cdf);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) c.cdf[b][t] = prop ? cdf[b] : 0.f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            c.o[d][t] = g.ro[3 * (int64_t)ray + d];
+            c.d[d][t] = g.rd[3 * (int64_t)ray + d];
+        }
+    }
+}
+
+// sq for the window [w0, w0 + wn) of the chunk's samples (needs beg[] complete).
+__device__ void expand_window(ChunkCtx& c, int32_t nseg, int32_t w0, int32_t wn)
+{
+    for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x)
+    {
+        const int32_t a = max(c.beg[t], w0), b = min(c.beg[t + 1], w0 + wn);
+        for (int32_t i = a; i < b; ++i) c.sq[i - w0] = (uint16_t)t;
+    }
+}
+
+// Sample j of chunk segment t: the same operations as sample_geo on the same values, so the same bits.
+__device__ __forceinline__ Sample ctx_sample(const ProjGeom& g, const ChunkCtx& c, int32_t t, int32_t j,
+                                             const float (&mn)[3], const float (&mx)[3])
+{
+    Sample s;
+    const int64_t ray  = c.ray[t];
+    const int32_t rank = c.rank[t], n = c.n[t];
+    const float tmin = c.tmin[t], tmax = c.tmax[t], step = c.step[t];
+    float cdf[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) cdf[b] = c.cdf[b][t];
+    float tt;
+    sample_tw(g, cdf[7] > 0.f, cdf, tmin, tmax, step, n, j, ray, rank, tt, s.w);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        float pd = c.o[d][t] + c.d[d][t] * tt;
+        s.uv[d]  = (pd - mn[d]) / (mx[d] - mn[d]) * 2 - 1;
+    }
+    s.ray = ray;
+    return s;
+}
+
+__device__ __forceinline__ void node_bounds(const ProjGeom& g, int32_t node, float (&mn)[3], float (&mx)[3])
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        mn[d] = g.pmin[3 * node + d];
+        mx[d] = g.pmax[3 * node + d];
+    }
+}
+
+// Forward: persistent workgroups of 4 waves over the chunks; tiles of 32 samples per wave on the matrix cores;
+// per-segment sums sequential in j as in k_proj_fwd.
 template <typename P, int H, int ACT>
 __global__ void __launch_bounds__(256) k_mfma_fwd(ProjGeom g, Plan p, const float* __restrict__ params,
-                                                  const void* grid_lowp, float* seg_partial)
+                                                  const void* grid_lowp, float* seg_partial, float* sample_out)
 {
     __shared__ WeightLDS<P, H> w;
+    __shared__ ChunkCtx cx;
     __shared__ float s_contrib[kFwdChunk];
-    __shared__ float s_seg[kFwdChunk];
-    const int64_t c = blockIdx.x;
-    if (c >= p.chunk_off1[g.N]) return;
+    __shared__ float s_seg[kSegCap];
     load_weights<P, H, 8>(params, w);
     const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
-    const int32_t slot = p.chunk1_slot[c];
-    const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c];
-    const int64_t base  = p.nm_soff[q0];
-    const int64_t nsamp = p.nm_soff[q1] - base;
-    const void* ngrid   = node_grid_ptr(g, grid_lowp, P::kBF16, slot);
-    for (int32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) s_seg[q - q0] = 0.f;
-    __syncthreads();
-    for (int64_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
+    const int64_t nchunks = p.chunk_off1[g.N];
+    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x)
     {
-        const int32_t wn = (int32_t)min<int64_t>(kFwdChunk, nsamp - w0);
-        for (int32_t t0 = wave * 32; t0 < wn; t0 += 128)
+        const int32_t slot = p.chunk1_slot[c];
+        const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c], nseg = q1 - q0;
+        if (nseg == 0) continue;
+        const int64_t base = p.nm_soff[q0];
+        __syncthreads();  // the previous chunk's readers of cx / s_seg are done
+        load_chunk_ctx(g, p, q0, q1, base, nullptr, cx);
+        for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x) s_seg[t] = 0.f;
+        float mn[3], mx[3];
+        node_bounds(g, g.seg_node[p.perm[q0]], mn, mx);
+        const void* ngrid = node_grid_ptr(g, grid_lowp, P::kBF16, slot);
+        __syncthreads();
+        const int32_t nsamp = cx.beg[nseg];
+        for (int32_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
         {
-            const int32_t i  = t0 + n;
-            const bool valid = i < wn;
-            float x4[4]      = {0.f, 0.f, 0.f, 0.f};
-            float wgt        = 0.f;
-            if (valid)
+            const int32_t wn = min(kFwdChunk, nsamp - w0);
+            expand_window(cx, nseg, w0, wn);
+            __syncthreads();
+            for (int32_t t0 = wave * 32; t0 < wn; t0 += 128)
             {
-                const int64_t s = w0 + i;
-                const int32_t q = find_segment(p.nm_soff, q0, q1, base + s);
-                Sample smp      = sample_geo(g, p.perm[q], (int32_t)(base + s - p.nm_soff[q]));
-                Corners cr      = corners_cl(smp.uv, g.G, 8);
-                gather_half<P::kBF16>(ngrid, cr, hg, x4);
-                wgt = smp.w;
+                const int32_t i  = t0 + n;
+                const bool valid = i < wn;
+                float x4[4]      = {0.f, 0.f, 0.f, 0.f};
+                float wgt        = 0.f;
+                if (valid)
+                {
+                    const int32_t t = cx.sq[i];
+                    Sample smp      = ctx_sample(g, cx, t, w0 + i - cx.beg[t], mn, mx);
+                    Corners cr      = corners_cl(smp.uv, g.G, 8);
+                    gather_half<P::kBF16>(ngrid, cr, hg, x4);
+                    wgt = smp.w;
+                }
+                Fwd<H / 32> f;
+                tile_forward_mfma<P, H, ACT>(w, x4, g.beta, f);
+                if (valid && hg == 0)
+                {
+                    const float y = act_f(NEAT_ACT_SOFTPLUS, f.z, g.beta);
+                    s_contrib[i]  = y * wgt;
+                    if (sample_out) sample_out[base + w0 + i] = y;
+                }
             }
-            Fwd<H / 32> f;
-            tile_forward_mfma<P, H, ACT>(w, x4, g.beta, f);
-            if (valid && hg == 0) s_contrib[i] = act_f(NEAT_ACT_SOFTPLUS, f.z, g.beta) * wgt;
+            __syncthreads();
+            for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x)
+            {
+                const int32_t a = max(cx.beg[t], w0), b = min(cx.beg[t + 1], w0 + wn);
+                if (a >= b) continue;
+                float acc = 0.f;
+                for (int32_t i = a; i < b; ++i) acc += s_contrib[i - w0];
+                s_seg[t] += acc;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (int32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x)
-        {
-            const int64_t a = max<int64_t>(p.nm_soff[q] - base, w0);
-            const int64_t b = min<int64_t>(p.nm_soff[q + 1] - base, w0 + wn);
-            if (a >= b) continue;
-            float acc = 0.f;
-            for (int64_t s = a; s < b; ++s) acc += s_contrib[s - w0];
-            s_seg[q - q0] += acc;
-        }
-        __syncthreads();
+        for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x) seg_partial[cx.k[t]] = s_seg[t];
     }
-    for (int32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) seg_partial[p.perm[q]] = s_seg[q - q0];
 }
 
-__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+// Rows of the X staging tile: the 8 features, then the columns that turn the X product into the bias and output
+// gradients (E = dZ1 [x; 1]^T + dZ2 [e10]^T + h2 [dz e9]^T, with the B operand masked per product).
+constexpr int kXOnes1 = 8, kXDz = 9, kXOnes2 = 10, kXRows = 11;
 
-// A / B fragment of a weight-gradient product over the tile's samples, from a [row][sample] staging buffer: FP32 step
-// st covers samples 2st + l/32; BF16 step st covers samples 16st + 8(l/32) + j.
-template <typename P>
-__device__ __forceinline__ typename P::Frag stage_frag(const float* buf, int row, int st, int hg, bool zero)
-{
-    typename P::Frag f;
-    if constexpr (P::kBF16)
-    {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) f[j] = (__bf16)(zero ? 0.f : buf[row * 33 + 16 * st + 8 * hg + j]);
-    }
-    else
-    {
-        f = zero ? 0.f : buf[row * 33 + 2 * st + hg];
-    }
-    return f;
-}
-
-// Backward: per wave and tile, recompute the forward, back-propagate on the matrix cores, write dL/dfeature per
-// sample (for the LDS scatter) and accumulate the parameter gradient; every wave writes its own slice of the
+// Backward: persistent workgroups of 4 waves; per wave and tile of 32 samples, recompute the forward, back-propagate
+// on the matrix cores, write dL/dfeature per sample (for the LDS scatter), and accumulate the parameter gradient
+// with products over the tile's samples (staged through LDS as [row][sample]). Each wave writes its own slice of the
 // partials (summed later in a fixed order).
 template <typename P, int H, int ACT>
 __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const float* __restrict__ params,
@@ -621,145 +879,170 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
     constexpr int MT = H / 32, S = steps_per_tile<P>(), C = 8;
     constexpr int KSTEPS = 32 / (2 * P::KS);  // sample steps of a weight-gradient product (FP32 16, BF16 2)
     __shared__ WeightLDS<P, H> w;
-    __shared__ float stage[4][2][H * 33];
+    __shared__ ChunkCtx cx;
+    __shared__ __align__(16) float stage[4][(2 * H + kXRows) * kRS];
     load_weights<P, H, C>(params, w);
-    __syncthreads();
     const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
-    float* sA = stage[wave][0];
-    float* sB = stage[wave][1];
-    f32x16 dW2[MT][MT], dW1[MT];
-    f32x16 accb1[MT], accb2[MT], accw3[MT];
+    float* sA = stage[wave];
+    float* sB = sA + H * kRS;
+    float* sX = sB + H * kRS;
+    f32x16 dW2[MT][MT], E[MT];
 #pragma unroll
     for (int a = 0; a < MT; ++a)
     {
-        dW1[a] = accb1[a] = accb2[a] = accw3[a] = f32x16{};
+        E[a] = f32x16{};
 #pragma unroll
         for (int b = 0; b < MT; ++b) dW2[a][b] = f32x16{};
     }
-    float accb3 = 0.f;
+    float b3 = 0.f;
+    // the constant rows of the X tile
+    if (hg == 0)
+    {
+        sX[kXOnes1 * kRS + n] = 1.f;
+        sX[kXOnes2 * kRS + n] = 1.f;
+    }
     const int64_t nchunks = p.chunk_off1[g.N];
     for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x)
     {
         const int32_t slot = p.chunk1_slot[c];
-        const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c];
-        const int64_t base  = p.nm_soff[q0];
-        const int64_t nsamp = p.nm_soff[q1] - base;
-        const void* ngrid   = node_grid_ptr(g, grid_lowp, P::kBF16, slot);
-        for (int64_t t0 = wave * 32; t0 < nsamp; t0 += 128)
+        const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c], nseg = q1 - q0;
+        if (nseg == 0) continue;
+        const int64_t base = p.nm_soff[q0];
+        __syncthreads();
+        load_chunk_ctx(g, p, q0, q1, base, grad_proj, cx);
+        float mn[3], mx[3];
+        node_bounds(g, g.seg_node[p.perm[q0]], mn, mx);
+        const void* ngrid = node_grid_ptr(g, grid_lowp, P::kBF16, slot);
+        __syncthreads();
+        const int32_t nsamp = cx.beg[nseg];
+        for (int32_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
         {
-            const int64_t s  = t0 + n;
-            const bool valid = s < nsamp;
-            float x4[4]      = {0.f, 0.f, 0.f, 0.f};
-            float gy         = 0.f;
-            if (valid)
+            const int32_t wn = min(kFwdChunk, nsamp - w0);
+            expand_window(cx, nseg, w0, wn);
+            __syncthreads();
+            for (int32_t t0 = wave * 32; t0 < wn; t0 += 128)
             {
-                const int32_t q = find_segment(p.nm_soff, q0, q1, base + s);
-                Sample smp      = sample_geo(g, p.perm[q], (int32_t)(base + s - p.nm_soff[q]));
-                Corners cr      = corners_cl(smp.uv, g.G, C);
-                gather_half<P::kBF16>(ngrid, cr, hg, x4);
-                gy = grad_proj[smp.ray] * smp.w;
-            }
-            Fwd<MT> f;
-            tile_forward_mfma<P, H, ACT>(w, x4, g.beta, f);
-            const float dz = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, f.z, g.beta) : 0.f;
-            if (hg == 0) accb3 += dz;
-            f32x16 dZ2[MT];
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-            {
-#pragma unroll
-                for (int i = 0; i < 16; ++i)
-                {
-                    const int r    = 32 * mt + crow(i, hg);
-                    const float zz = f.z2[mt][i];
-                    dZ2[mt][i]     = w.vec[2][r] * dz * act_d(ACT, zz, g.beta);
-                    accw3[mt][i] += dz * act_f(ACT, zz, g.beta);
-                    accb2[mt][i] += dZ2[mt][i];
-                }
-            }
-            f32x16 dZ1[MT];
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-            {
-                f32x16 acc = {};
-#pragma unroll
-                for (int ms = 0; ms < MT; ++ms)
-                {
-#pragma unroll
-                    for (int st = 0; st < S; ++st)
-                        acc = P::mma(w.w2t[mt][ms][st][l], P::from_acc(dZ2[ms], st, [](float v, int) { return v; }),
-                                     acc);
-                }
-#pragma unroll
-                for (int i = 0; i < 16; ++i)
-                {
-                    acc[i] *= act_d(ACT, f.z1[mt][i], g.beta);
-                    accb1[mt][i] += acc[i];
-                }
-                dZ1[mt] = acc;
-            }
-            // dL/dfeature: rows 0..7 of W0^T dZ1, i.e. registers 0..3 of each lane group (channels 4g..4g+3)
-            {
-                f32x16 dx = {};
-#pragma unroll
-                for (int ms = 0; ms < MT; ++ms)
-                {
-#pragma unroll
-                    for (int st = 0; st < S; ++st)
-                        dx = P::mma(w.w1t[ms][st][l], P::from_acc(dZ1[ms], st, [](float v, int) { return v; }), dx);
-                }
+                const int32_t i  = t0 + n;
+                const bool valid = i < wn;
+                float x4[4]      = {0.f, 0.f, 0.f, 0.f};
+                float gy         = 0.f;
                 if (valid)
-                    *reinterpret_cast<float4*>(dfeat + (base + s) * C + 4 * hg) = make_float4(dx[0], dx[1], dx[2], dx[3]);
-            }
-            // dW(hidden) += dZ2 act(Z1)^T over the tile's samples
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-            {
-#pragma unroll
-                for (int i = 0; i < 16; ++i)
                 {
-                    const int r    = 32 * mt + crow(i, hg);
-                    sA[r * 33 + n] = dZ2[mt][i];
-                    sB[r * 33 + n] = act_f(ACT, f.z1[mt][i], g.beta);
+                    const int32_t t = cx.sq[i];
+                    Sample smp      = ctx_sample(g, cx, t, w0 + i - cx.beg[t], mn, mx);
+                    Corners cr      = corners_cl(smp.uv, g.G, C);
+                    gather_half<P::kBF16>(ngrid, cr, hg, x4);
+                    gy = cx.gproj[t] * smp.w;
                 }
-            }
-            wave_lds_fence();
+                Fwd<MT> f;
+                tile_forward_mfma<P, H, ACT>(w, x4, g.beta, f);
+                const float dz = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, f.z, g.beta) : 0.f;
+                if (hg == 0) b3 += dz;
+                // X tile and the first staged pair: dZ2 (A) and act(z1) (B)
 #pragma unroll
-            for (int st = 0; st < KSTEPS; ++st)
-            {
+                for (int j = 0; j < 4; ++j) sX[(4 * hg + j) * kRS + n] = x4[j];
+                if (hg == 0) sX[kXDz * kRS + n] = dz;
+                f32x16 dZ2[MT];
 #pragma unroll
-                for (int m2 = 0; m2 < MT; ++m2)
+                for (int mt = 0; mt < MT; ++mt)
                 {
-                    auto a = stage_frag<P>(sA, 32 * m2 + n, st, hg, false);
 #pragma unroll
-                    for (int m1 = 0; m1 < MT; ++m1)
-                        dW2[m2][m1] = P::mma(a, stage_frag<P>(sB, 32 * m1 + n, st, hg, false), dW2[m2][m1]);
+                    for (int i2 = 0; i2 < 16; ++i2)
+                    {
+                        const int r    = 32 * mt + crow(i2, hg);
+                        dZ2[mt][i2]    = w.vec[2][r] * dz * act_d(ACT, f.z2[mt][i2], g.beta);
+                        sA[r * kRS + n] = dZ2[mt][i2];
+                        sB[r * kRS + n] = act_f(ACT, f.z1[mt][i2], g.beta);
+                    }
                 }
+                wave_lds_fence();
+                // dW(hidden) += dZ2 act(z1)^T; E += dZ2 e10^T (the hidden bias gradient, column 10)
+#pragma unroll
+                for (int st = 0; st < KSTEPS; ++st)
+                {
+                    const auto bx = stage_frag<P>(sX, n, st, hg, n != kXOnes2);
+#pragma unroll
+                    for (int m2 = 0; m2 < MT; ++m2)
+                    {
+                        const auto a = stage_frag<P>(sA, 32 * m2 + n, st, hg, false);
+                        E[m2]        = P::mma(a, bx, E[m2]);
+#pragma unroll
+                        for (int m1 = 0; m1 < MT; ++m1)
+                            dW2[m2][m1] = P::mma(a, stage_frag<P>(sB, 32 * m1 + n, st, hg, false), dW2[m2][m1]);
+                    }
+                }
+                wave_lds_fence();
+                // E += act(z2) (dz e9)^T: the output-layer weight gradient, column 9
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+#pragma unroll
+                    for (int i2 = 0; i2 < 16; ++i2)
+                        sB[(32 * mt + crow(i2, hg)) * kRS + n] = act_f(ACT, f.z2[mt][i2], g.beta);
+                }
+                wave_lds_fence();
+#pragma unroll
+                for (int st = 0; st < KSTEPS; ++st)
+                {
+                    const auto bx = stage_frag<P>(sX, n, st, hg, n != kXDz);
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt) E[mt] = P::mma(stage_frag<P>(sB, 32 * mt + n, st, hg, false), bx, E[mt]);
+                }
+                // dZ1 = (W1^T dZ2) * act'(z1), in registers
+                f32x16 dZ1[MT];
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+                    f32x16 acc = {};
+#pragma unroll
+                    for (int ms = 0; ms < MT; ++ms)
+                    {
+#pragma unroll
+                        for (int st = 0; st < S; ++st)
+                            acc = P::mma(w.w2t[mt][ms][st][l], P::from_acc(dZ2[ms], st, [](float v, int) { return v; }),
+                                         acc);
+                    }
+#pragma unroll
+                    for (int i2 = 0; i2 < 16; ++i2) acc[i2] *= act_d(ACT, f.z1[mt][i2], g.beta);
+                    dZ1[mt] = acc;
+                }
+                // dL/dfeature: rows 0..7 of W0^T dZ1, i.e. registers 0..3 of each lane group (channels 4g..4g+3)
+                {
+                    f32x16 dx = {};
+#pragma unroll
+                    for (int ms = 0; ms < MT; ++ms)
+                    {
+#pragma unroll
+                        for (int st = 0; st < S; ++st)
+                            dx = P::mma(w.w1t[ms][st][l], P::from_acc(dZ1[ms], st, [](float v, int) { return v; }), dx);
+                    }
+                    if (valid)
+                        *reinterpret_cast<float4*>(dfeat + (base + w0 + i) * C + 4 * hg) =
+                            make_float4(dx[0], dx[1], dx[2], dx[3]);
+                }
+                // E += dZ1 [x; 1]^T: the first-layer weight gradient (columns 0..7) and bias gradient (column 8)
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+#pragma unroll
+                    for (int i2 = 0; i2 < 16; ++i2) sA[(32 * mt + crow(i2, hg)) * kRS + n] = dZ1[mt][i2];
+                }
+                wave_lds_fence();
+#pragma unroll
+                for (int st = 0; st < KSTEPS; ++st)
+                {
+                    const auto bx = stage_frag<P>(sX, n, st, hg, n > kXOnes1);
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt) E[mt] = P::mma(stage_frag<P>(sA, 32 * mt + n, st, hg, false), bx, E[mt]);
+                }
+                wave_lds_fence();
             }
-            wave_lds_fence();
-            // dW(first) += dZ1 x^T over the tile's samples (N padded from 8 channels to 32)
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-            {
-#pragma unroll
-                for (int i = 0; i < 16; ++i) sA[(32 * mt + crow(i, hg)) * 33 + n] = dZ1[mt][i];
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) sB[(4 * hg + j) * 33 + n] = x4[j];
-            wave_lds_fence();
-#pragma unroll
-            for (int st = 0; st < KSTEPS; ++st)
-            {
-                auto b = stage_frag<P>(sB, n, st, hg, n >= C);
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt) dW1[mt] = P::mma(stage_frag<P>(sA, 32 * mt + n, st, hg, false), b, dW1[mt]);
-            }
-            wave_lds_fence();
+            __syncthreads();
         }
     }
     // ---- this wave's slice of the partial parameter gradient (packed layout) ----
-    float* part     = partials + ((int64_t)blockIdx.x * 4 + wave) * P_total;
-    const int oW0   = 0, ob0 = H * C, oW1 = ob0 + H, ob1 = oW1 + H * H, oW2 = ob1 + H, ob2 = oW2 + H;
+    float* part   = partials + ((int64_t)blockIdx.x * 4 + wave) * P_total;
+    const int oW0 = 0, ob0 = H * C, oW1 = ob0 + H, ob1 = oW1 + H * H, oW2 = ob1 + H, ob2 = oW2 + H;
 #pragma unroll
     for (int m2 = 0; m2 < MT; ++m2)
     {
@@ -767,41 +1050,22 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
         for (int m1 = 0; m1 < MT; ++m1)
         {
 #pragma unroll
-            for (int i = 0; i < 16; ++i)
-                part[oW1 + (32 * m2 + crow(i, hg)) * H + 32 * m1 + n] = dW2[m2][m1][i];
+            for (int i2 = 0; i2 < 16; ++i2) part[oW1 + (32 * m2 + crow(i2, hg)) * H + 32 * m1 + n] = dW2[m2][m1][i2];
         }
     }
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
     {
 #pragma unroll
-        for (int i = 0; i < 16; ++i)
-            if (n < C) part[oW0 + (32 * mt + crow(i, hg)) * C + n] = dW1[mt][i];
-    }
-    // per-row sums over the 32 samples of a lane group: through the staging buffer
-    wave_lds_fence();
-    for (int v = 0; v < 3; ++v)
-    {
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
+        for (int i2 = 0; i2 < 16; ++i2)
         {
-#pragma unroll
-            for (int i = 0; i < 16; ++i)
-            {
-                const float val = v == 0 ? accb1[mt][i] : (v == 1 ? accb2[mt][i] : accw3[mt][i]);
-                sA[(32 * mt + crow(i, hg)) * 33 + n] = val;
-            }
+            const int r = 32 * mt + crow(i2, hg);
+            if (n < C) part[oW0 + r * C + n] = E[mt][i2];
+            if (n == kXOnes1) part[ob0 + r] = E[mt][i2];
+            if (n == kXDz) part[oW2 + r] = E[mt][i2];
+            if (n == kXOnes2) part[ob1 + r] = E[mt][i2];
         }
-        wave_lds_fence();
-        for (int r = l; r < H; r += 64)
-        {
-            float acc = 0.f;
-            for (int k = 0; k < 32; ++k) acc += sA[r * 33 + k];
-            part[(v == 0 ? ob0 : (v == 1 ? ob1 : oW2)) + r] = acc;
-        }
-        wave_lds_fence();
     }
-    float b3 = accb3;
 #pragma unroll
     for (int off = 16; off >= 1; off >>= 1) b3 += __shfl_xor(b3, off);
     if (l == 0) part[ob2] = b3;
@@ -841,7 +1105,7 @@ int32_t scatter_chunk(int64_t S)
 ProjSizes proj_sizes(int64_t K, int64_t S, int64_t N, int32_t G, int32_t C, int32_t P)
 {
     ProjSizes z{K, S, N, G, C, P, 0, 0, scatter_chunk(S), 0};
-    z.max1 = (S + kFwdChunk - 1) / kFwdChunk + N;
+    z.max1 = (S + (int64_t)kSegCost * K + kFwdChunk - 1) / kFwdChunk + N;
     z.max2 = (S + z.chunk2 - 1) / z.chunk2 + N;
     size_t tmp = 0;
     hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
@@ -873,7 +1137,8 @@ Layout proj_layout(const ProjSizes& z)
 
 int bw1_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 2048)); }
 // matrix-core backward: persistent workgroups of 4 waves, one partial slice per wave
-int mfma_bwd_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 512)); }
+int mfma_bwd_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 256)); }
+int mfma_fwd_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 1024)); }
 
 size_t partials_bytes(const ProjSizes& z)
 {
@@ -959,6 +1224,9 @@ int make_args(Args& a, const neat_projector_args* x)
                     "relu/silu hidden activations and a softplus output (got in=%d C=%d hidden=%d layers=%d out=%d "
                     "act=%d last=%d)",
                     x->in_features, C, x->hidden, x->hidden_layers, x->out_features, x->act, x->last_act);
+    if ((int64_t)G * G * G * C * 4 > 160 * 1024)
+        return fail(NEAT_E_UNSUPPORTED, "projector: a node grid (%d^3 x %d FP32) exceeds the 160 KiB of LDS the "
+                    "gradient scatter holds it in", G, C);
     if (x->precision == NEAT_PREC_BF16)
     {
         if (C != 8) return fail(NEAT_E_UNSUPPORTED, "projector: the BF16 kernels need 8 channels (got %d)", C);
@@ -966,9 +1234,10 @@ int make_args(Args& a, const neat_projector_args* x)
                      "projector: NEAT_PREC_BF16 needs a 16-byte aligned grid_bf16");
     }
     NEAT_REQUIRE((reinterpret_cast<uintptr_t>(x->grid) % 16) == 0, "projector: grid must be 16-byte aligned");
+    NEAT_REQUIRE((reinterpret_cast<uintptr_t>(x->seg_cdf) % 16) == 0, "projector: seg_cdf must be 16-byte aligned");
     a.g = ProjGeom{x->node_pos_min, x->node_pos_max, x->node_active_prefix_sum, x->num_nodes, x->ray_origin,
                    x->ray_direction, R, x->ray_seg_offset, K, x->seg_node, x->seg_ray, x->seg_count, x->seg_tmin,
-                   x->seg_tmax, x->jitter, x->seed, x->grid, N, G, x->softplus_beta};
+                   x->seg_tmax, x->jitter, x->seed, x->grid, N, G, x->softplus_beta, x->seg_cdf};
     a.m         = MLP{x->in_features, x->hidden, x->hidden_layers, x->out_features};
     a.act       = x->act;
     a.params    = x->params;
@@ -1002,13 +1271,13 @@ int launch_fwd_valu(hipStream_t s, const Args& a, const Plan& p, float* seg_part
 }
 
 template <typename P, int H>
-int launch_fwd_mfma(hipStream_t s, const Args& a, const Plan& p, float* seg_partial)
+int launch_fwd_mfma(hipStream_t s, const Args& a, const Plan& p, float* seg_partial, float* sample_out = nullptr)
 {
-    unsigned grid = (unsigned)std::max<int64_t>(1, a.z.max1);
+    const unsigned grid = (unsigned)mfma_fwd_workgroups(a.z);
     if (a.act == NEAT_ACT_RELU)
-        k_mfma_fwd<P, H, NEAT_ACT_RELU><<<grid, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, seg_partial);
+        k_mfma_fwd<P, H, NEAT_ACT_RELU><<<grid, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, seg_partial, sample_out);
     else
-        k_mfma_fwd<P, H, NEAT_ACT_SILU><<<grid, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, seg_partial);
+        k_mfma_fwd<P, H, NEAT_ACT_SILU><<<grid, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, seg_partial, sample_out);
     NEAT_LAUNCHED();
     return NEAT_OK;
 }
@@ -1022,8 +1291,8 @@ int launch_scatter(hipStream_t s, const Args& a, const Plan& p, const float* dfe
     const size_t smem   = (size_t)gsize * sizeof(float);
     NEAT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_proj_bw2<C>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-    k_proj_bw2<C><<<(unsigned)std::max<int64_t>(1, a.z.max2), 256, smem, s>>>(a.g, p, dfeat, grad_grid,
-                                                                               grid_partials);
+    k_proj_bw2<C><<<(unsigned)std::max<int64_t>(1, a.z.max2), kScatterThreads, smem, s>>>(a.g, p, dfeat, grad_grid,
+                                                                                           grid_partials);
     NEAT_LAUNCHED();
     dim3 rg(grid_for(gsize, 256), (unsigned)a.g.N);
     k_proj_bw2_reduce<<<rg, 256, 0, s>>>(a.g, p, gsize, grid_partials, grad_grid);
@@ -1097,6 +1366,14 @@ int dispatch_bwd(hipStream_t s, const Args& a, const Plan& p, const float* gproj
                        : launch_bw_valu<8, 32>(s, a, p, gproj, wt, dfeat, partials, gpart, ggrid, gparams);
     return H == 64 ? launch_bw_valu<4, 64>(s, a, p, gproj, wt, dfeat, partials, gpart, ggrid, gparams)
                    : launch_bw_valu<4, 32>(s, a, p, gproj, wt, dfeat, partials, gpart, ggrid, gparams);
+}
+
+// first node-major sample of every canonical segment
+__global__ void __launch_bounds__(256) k_seg_value_offset(const int32_t* perm, const int64_t* nm_soff, int64_t K,
+                                                          int64_t* off)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < K) off[perm[q]] = nm_soff[q];
 }
 
 __global__ void __launch_bounds__(256) k_to_bf16(const float* src, int64_t n, unsigned short* dst)
@@ -1194,6 +1471,35 @@ extern "C"
                                                                       compensated, proj);
             NEAT_LAUNCHED();
         }
+        return NEAT_OK;
+    }
+
+    int neat_projector_sample_values(neat_stream_t stream, const neat_projector_args* x, float* values,
+                                     int64_t* seg_value_offset, void* ws, size_t bytes)
+    {
+        Args a;
+        int st = make_args(a, x);
+        if (st) return st;
+        if ((st = check_ws(a, ws, bytes))) return st;
+        if (!a.mfma)
+            return fail(NEAT_E_UNSUPPORTED, "neat_projector_sample_values: needs the matrix-core kernels (8 channels, "
+                                            "precision fp32 or bf16)");
+        NEAT_REQUIRE(a.z.K == 0 || (values && seg_value_offset), "neat_projector_sample_values: null output");
+        if (a.z.K == 0) return NEAT_OK;
+        hipStream_t s      = as_stream(stream);
+        Plan p             = plan_view(ws, a.z);
+        float* seg_partial = reinterpret_cast<float*>(region(ws, a.z, 11));
+        k_fill_f32<<<grid_for(a.z.K, 256), 256, 0, s>>>(seg_partial, a.z.K, 0.f);
+        NEAT_LAUNCHED();
+        if (a.precision == NEAT_PREC_BF16)
+            st = a.m.hidden == 64 ? launch_fwd_mfma<PrecBF16, 64>(s, a, p, seg_partial, values)
+                                  : launch_fwd_mfma<PrecBF16, 32>(s, a, p, seg_partial, values);
+        else
+            st = a.m.hidden == 64 ? launch_fwd_mfma<PrecF32, 64>(s, a, p, seg_partial, values)
+                                  : launch_fwd_mfma<PrecF32, 32>(s, a, p, seg_partial, values);
+        if (st) return st;
+        k_seg_value_offset<<<grid_for(a.z.K, 256), 256, 0, s>>>(p.perm, p.nm_soff, a.z.K, seg_value_offset);
+        NEAT_LAUNCHED();
         return NEAT_OK;
     }
 

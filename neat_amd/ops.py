@@ -191,6 +191,12 @@ class Projector:
         a.num_segments = K
         for f in ("seg_node", "seg_ray", "seg_count", "seg_tmin", "seg_tmax"):
             setattr(a, f, segs[f].data_ptr())
+        if segs.get("seg_cdf") is not None:
+            cdf = segs["seg_cdf"]
+            if cdf.shape != (K, 8) or cdf.dtype != _F32 or not cdf.is_contiguous():
+                raise ValueError("seg_cdf must be a contiguous float32 [num_segments, 8] tensor")
+            self._keep.append(cdf)
+            a.seg_cdf = cdf.data_ptr()
         a.num_samples, a.jitter, a.seed = S, int(bool(jitter)), seed
         a.grid, a.num_slots, a.channels, a.grid_size = grid.data_ptr(), N, C, G
         for i, st in enumerate(grid.stride()):
@@ -222,6 +228,14 @@ class Projector:
              ptr(self.ws), self.ws.numel())
         return proj
 
+    def sample_values(self):
+        """(values [S] in the plan's node-major order, seg_value_offset [K]): the decoded field at every sample."""
+        values = torch.empty(max(self.S, 1), dtype=_F32, device=self.grid.device)
+        off = torch.empty(max(self.args.num_segments, 1), dtype=torch.int64, device=self.grid.device)
+        call("neat_projector_sample_values", stream_ptr(self.stream), ctypes.byref(self.args), ptr(values), ptr(off),
+             ptr(self.ws), self.ws.numel())
+        return values, off
+
     def backward(self, grad_proj):
         grad_proj = grad_proj.contiguous()
         gg = torch.empty_like(self.grid)
@@ -229,3 +243,51 @@ class Projector:
         call("neat_projector_bwd", stream_ptr(self.stream), ctypes.byref(self.args), ptr(grad_proj), ptr(gg), ptr(gp),
              ptr(self.ws), self.ws.numel())
         return gg, gp
+
+
+def adam_step(param, grad, exp_avg, exp_avg_sq, lr, step, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0,
+              stream=None):
+    """One in-place Adam step (neat_adam_step) on dense FP32 GPU tensors of one layout."""
+    for t in (grad, exp_avg, exp_avg_sq):
+        if t.shape != param.shape or t.stride() != param.stride() or t.dtype != _F32:
+            raise ValueError("adam_step: every buffer must match the parameter's shape, strides and dtype")
+    call("neat_adam_step", stream_ptr(stream), ptr(param), ptr(grad), ptr(exp_avg), ptr(exp_avg_sq), param.numel(),
+         float(lr), float(beta1), float(beta2), float(eps), float(weight_decay), int(step))
+
+
+# ---- learned proposal --------------------------------------------------------------------------------------------
+def proposal_num_params():
+    return int(_abi.lib().neat_proposal_num_params())
+
+
+def proposal_fwd(origin, direction, segs, params, num_samples=8, want_logits=False, stream=None):
+    """(seg_cdf [K, 8], seg_count_out [K] int32, logits or None) for a canonical segment list (neat_proposal_fwd)."""
+    K = segs["seg_ray"].shape[0]
+    dev = origin.device
+    cdf = torch.empty((K, 8), dtype=_F32, device=dev)
+    cnt = torch.empty(K, dtype=torch.int32, device=dev)
+    lg = torch.empty((K, 8), dtype=_F32, device=dev) if want_logits else None
+    call("neat_proposal_fwd", stream_ptr(stream), ptr(origin), ptr(direction), ptr(segs["seg_ray"]),
+         ptr(segs["seg_tmin"]), ptr(segs["seg_tmax"]), ptr(segs["seg_count"]), K, ptr(params), int(num_samples),
+         ptr(cdf), ptr(cnt), ptr(lg) if lg is not None else None)
+    return cdf, cnt, lg
+
+
+def proposal_targets(values, seg_value_offset, seg_count, stream=None):
+    K = seg_count.shape[0]
+    q = torch.empty((K, 8), dtype=_F32, device=seg_count.device)
+    call("neat_proposal_targets", stream_ptr(stream), ptr(values), ptr(seg_value_offset), ptr(seg_count), K, ptr(q))
+    return q
+
+
+def proposal_bwd(origin, direction, segs, params, target, scale=None, stream=None):
+    """Gradient of scale * sum_k CE(target_k, softmax(logits_k)) (default scale: 1 / K, the mean)."""
+    K = segs["seg_ray"].shape[0]
+    scale = 1.0 / max(K, 1) if scale is None else scale
+    gp = torch.empty_like(params)
+    nbytes = int(_abi.lib().neat_proposal_workspace_bytes(K))
+    ws = torch.empty(max(nbytes, 16), dtype=torch.uint8, device=params.device)
+    call("neat_proposal_bwd", stream_ptr(stream), ptr(origin), ptr(direction), ptr(segs["seg_ray"]),
+         ptr(segs["seg_tmin"]), ptr(segs["seg_tmax"]), K, ptr(params), ptr(target), float(scale), ptr(gp), ptr(ws),
+         ws.numel())
+    return gp

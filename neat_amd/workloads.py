@@ -195,6 +195,9 @@ C1 = Config("C1", depth=2, adaptive_leaves=0, grid=11, channels=8, hidden=32, la
             ellipsoids=20, batch=1 << 18)
 C2 = Config("C2", depth=5, adaptive_leaves=4096, grid=17, channels=8, hidden=64, layers=1, views=256, det=512,
             max_spn=64, ellipsoids=100, batch=1 << 18)
+# training form: C1 geometry, rays drawn from 256 views; Adam at lr 1e-2 (grid) and 1e-3 (decoder)
+C4 = Config("C4", depth=2, adaptive_leaves=0, grid=11, channels=8, hidden=32, layers=1, views=256, det=128,
+            max_spn=32, ellipsoids=20, batch=1 << 18, extra=dict(lr_grid=1e-2, lr_decoder=1e-3))
 
 
 def make_scene(cfg, seed=0, views=None, det=None, phantom_seed=None):
@@ -225,3 +228,29 @@ def batch_rays(cfg, batch, seed=0, det=None):
     idx = np.sort(rng.choice(total, batch, replace=False))
     v, pix = idx // (det * det), idx % (det * det)
     return pixel_rays(cfg.views, det, det, v, pix // det, pix % det, det_extent=cfg.det_extent)
+
+
+def phantom_field(scene, cfg, floor=0.01):
+    """A grid and decoder whose decoded field is the phantom's density, trilinearly interpolated on every active
+    leaf's grid vertices: channel 0 holds softplus_beta^-1(density) (density clamped to >= floor), the other channels
+    0, and the decoder passes channel 0 through (relu(x) - relu(-x) in both hidden layers), so that
+    field = softplus_beta(x0). Used as a reconstructed-looking field (piecewise smooth, sharp at ellipsoid surfaces)
+    for the proposal experiments. Needs relu hidden units."""
+    assert cfg.act == "relu"
+    tree, ids, prefix = scene["tree"], scene["active_ids"], scene["prefix"]
+    G, C, H, beta = cfg.grid, cfg.channels, cfg.hidden, cfg.beta
+    grid = np.zeros_like(scene["grid"])
+    lin = np.arange(G) / (G - 1)
+    zz, yy, xx = np.meshgrid(lin, lin, lin, indexing="ij")
+    for node in ids:
+        mn, mx = tree["pos_min"][node].astype(np.float64), tree["pos_max"][node].astype(np.float64)
+        pts = np.stack([mn[0] + xx * (mx[0] - mn[0]), mn[1] + yy * (mx[1] - mn[1]), mn[2] + zz * (mx[2] - mn[2])], -1)
+        rho = np.maximum(scene["phantom"].density_at(pts.reshape(-1, 3)), floor)
+        grid[prefix[node], 0] = (np.log(np.expm1(beta * rho)) / beta).reshape(G, G, G)
+    P = []
+    W0 = np.zeros((H, C)); W0[0, 0], W0[1, 0] = 1, -1
+    W1 = np.zeros((H, H)); W1[0, 0], W1[1, 1] = 1, 1
+    W2 = np.zeros((1, H)); W2[0, 0], W2[0, 1] = 1, -1
+    for W in (W0, np.zeros(H), W1, np.zeros(H), W2, np.zeros(1)):
+        P.append(np.asarray(W).ravel())
+    return grid.astype(np.float32), np.concatenate(P).astype(np.float32)

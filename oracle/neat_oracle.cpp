@@ -502,7 +502,40 @@ struct SegList
     const float* seg_tmin;
     const float* seg_tmax;
     const int64_t* ray_seg_offset;  // to derive the segment rank within the ray
+    const float* seg_cdf;           // optional [K][8] proposal bin CDF; a row with cdf[7] == 0 is uniform
 };
+
+// ---------------------------------------------------------------------------------------------------------------
+// Proposal-guided sample placement (no reference counterpart; the build's own definition). A segment's proposal is
+// a piecewise-constant density over 8 equal bins of [tmin, tmax], given as its CDF (cdf[b] = P(bin <= b)). Its n
+// samples split [0, 1] into n equal-probability cells: cell j spans Q(j/n) .. Q((j+1)/n) of the segment (Q the
+// inverse CDF, the ends pinned to tmin and tmax), the sample sits at t1 (1 - a) + t2 a inside its cell (a = 0.5 or
+// the jitter, as for uniform samples) and its weight is the cell's length, so the weights sum to tmax - tmin and a
+// uniform CDF gives the uniform cells.
+// ---------------------------------------------------------------------------------------------------------------
+template <typename T>
+inline T inv_cdf8(const float* cdf, T u)
+{
+    int b = 0;
+    T lo  = 0;
+    while (b < 7 && (T)cdf[b] <= u)
+    {
+        lo = (T)cdf[b];
+        ++b;
+    }
+    T pb = (T)cdf[b] - lo;
+    T f  = pb > 0 ? (u - lo) / pb : (T)0;
+    f    = std::min<T>(std::max<T>(f, (T)0), (T)1);
+    return ((T)b + f) * (T)0.125;
+}
+
+template <typename T>
+inline T proposal_cell(const float* cdf, T tmin, T tmax, int32_t n, int32_t j)
+{
+    if (j <= 0) return tmin;
+    if (j >= n) return tmax;
+    return tmin + (tmax - tmin) * inv_cdf8<T>(cdf, (T)j / (T)n);
+}
 
 template <typename T>
 struct SampleCtx
@@ -550,6 +583,24 @@ inline void sample_uv(const SampleCtx<T>& c, int64_t ray, int64_t node, T t, T* 
     }
 }
 
+// t and weight of sample j: uniform cells (HyperTree.cu:339-389) or, with a proposal CDF, its cells.
+template <typename T>
+inline void sample_tw(const SampleCtx<T>& c, const float* cdf, T tmin, T tmax, T step, int32_t n, int32_t j,
+                      int64_t ray, int32_t rank, T& t, T& w)
+{
+    const T a = seg_a<T>(c.jitter, c.seed, ray, rank, j);
+    if (!cdf)
+    {
+        t = sample_t<T>(tmin, step, j, a);
+        w = sample_w<T>(tmin, tmax, step, n, j, c.jitter, c.seed, ray, rank);
+        return;
+    }
+    const T t1 = proposal_cell<T>(cdf, tmin, tmax, n, j);
+    const T t2 = proposal_cell<T>(cdf, tmin, tmax, n, j + 1);
+    t          = t1 * ((T)1 - a) + t2 * a;
+    w          = t2 - t1;
+}
+
 template <typename T, typename G>
 void projector_fwd(const SampleCtx<T>& c, const SegList& s, const GridDesc& g, const G* grid, const MLPDesc& m,
                    const float* params, int64_t num_rays, T* proj, int nthreads)
@@ -568,10 +619,11 @@ void projector_fwd(const SampleCtx<T>& c, const SegList& s, const GridDesc& g, c
             segment_params<T>(c, s, k, tmin, tmax, step, n, ray, rank);
             int64_t node = s.seg_node[k];
             int64_t slot = c.prefix[node];
+            const float* cdf = (s.seg_cdf && s.seg_cdf[8 * k + 7] > 0) ? s.seg_cdf + 8 * k : nullptr;
             for (int32_t j = 0; j < n; ++j)
             {
-                T t = sample_t<T>(tmin, step, j, seg_a<T>(c.jitter, c.seed, ray, rank, j));
-                T w = sample_w<T>(tmin, tmax, step, n, j, c.jitter, c.seed, ray, rank);
+                T t, w;
+                sample_tw<T>(c, cdf, tmin, tmax, step, n, j, ray, rank, t, w);
                 T uv[3], feat[64], y[4];
                 sample_uv<T>(c, ray, node, t, uv);
                 grid_sample_one<T, G>(g, grid, slot, uv[0], uv[1], uv[2], feat);
@@ -621,10 +673,11 @@ void projector_bwd(const SampleCtx<T>& c, const SegList& s, const GridDesc& g, c
                 segment_params<T>(c, s, k, tmin, tmax, step, n, ray, rank);
                 int64_t node = s.seg_node[k];
                 int64_t slot = c.prefix[node];
+                const float* cdf = (s.seg_cdf && s.seg_cdf[8 * k + 7] > 0) ? s.seg_cdf + 8 * k : nullptr;
                 for (int32_t j = 0; j < n; ++j)
                 {
-                    T t = sample_t<T>(tmin, step, j, seg_a<T>(c.jitter, c.seed, ray, rank, j));
-                    T w = sample_w<T>(tmin, tmax, step, n, j, c.jitter, c.seed, ray, rank);
+                    T t, w;
+                    sample_tw<T>(c, cdf, tmin, tmax, step, n, j, ray, rank, t, w);
                     T uv[3], feat[64], gfeat[64];
                     sample_uv<T>(c, ray, node, t, uv);
                     grid_sample_one<T, G>(g, grid, slot, uv[0], uv[1], uv[2], feat);
@@ -960,13 +1013,14 @@ extern "C"
     void oracle_projector_fwd(int32_t prec, const float* pos_min, const float* pos_max, const int64_t* prefix,
                               const float* ray_o, const float* ray_d, int64_t num_rays, const int64_t* ray_seg_offset,
                               int64_t num_segments, const int32_t* seg_node, const int32_t* seg_ray,
-                              const int32_t* seg_count, const float* seg_tmin, const float* seg_tmax, int32_t jitter,
+                              const int32_t* seg_count, const float* seg_tmin, const float* seg_tmax,
+                              const float* seg_cdf, int32_t jitter,
                               uint64_t seed, const float* grid, int64_t N, int32_t C, int32_t ID, int32_t IH,
                               int32_t IW, const int64_t* strides, int32_t in, int32_t hidden, int32_t layers,
                               int32_t out, const float* params, int32_t act, int32_t last_act, float beta,
                               double* proj, int32_t nthreads)
     {
-        SegList s{num_segments, seg_node, seg_ray, seg_count, seg_tmin, seg_tmax, ray_seg_offset};
+        SegList s{num_segments, seg_node, seg_ray, seg_count, seg_tmin, seg_tmax, ray_seg_offset, seg_cdf};
         GridDesc g = make_grid(N, C, ID, IH, IW, strides);
         MLPDesc m{in, hidden, layers, out, act, last_act, beta};
         if (prec == 0)
@@ -988,14 +1042,15 @@ extern "C"
     void oracle_projector_bwd(int32_t prec, const float* pos_min, const float* pos_max, const int64_t* prefix,
                               const float* ray_o, const float* ray_d, int64_t num_rays, const int64_t* ray_seg_offset,
                               int64_t num_segments, const int32_t* seg_node, const int32_t* seg_ray,
-                              const int32_t* seg_count, const float* seg_tmin, const float* seg_tmax, int32_t jitter,
+                              const int32_t* seg_count, const float* seg_tmin, const float* seg_tmax,
+                              const float* seg_cdf, int32_t jitter,
                               uint64_t seed, const float* grid, int64_t N, int32_t C, int32_t ID, int32_t IH,
                               int32_t IW, const int64_t* strides, int64_t grid_numel, int32_t in, int32_t hidden,
                               int32_t layers, int32_t out, const float* params, int32_t act, int32_t last_act,
                               float beta, const double* grad_proj, double* grad_grid, double* grad_params,
                               int32_t nthreads)
     {
-        SegList s{num_segments, seg_node, seg_ray, seg_count, seg_tmin, seg_tmax, ray_seg_offset};
+        SegList s{num_segments, seg_node, seg_ray, seg_count, seg_tmin, seg_tmax, ray_seg_offset, seg_cdf};
         GridDesc g = make_grid(N, C, ID, IH, IW, strides);
         MLPDesc m{in, hidden, layers, out, act, last_act, beta};
         int64_t P  = m.nparams();

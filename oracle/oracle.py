@@ -54,10 +54,10 @@ def lib():
             "oracle_decoder_bwd_f64": (None, [_P, _I64, _I32, _I32, _I32, _I32, _P, _I32, _I32, _F, _P, _P, _P]),
             "oracle_decoder_bwd_f32": (None, [_P, _I64, _I32, _I32, _I32, _I32, _P, _I32, _I32, _F, _P, _P, _P]),
             "oracle_integrate_fwd": (None, [_P, _P, _P, _I64, _I32, _I64, _P]),
-            "oracle_projector_fwd": (None, [_I32, _P, _P, _P, _P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _I32,
+            "oracle_projector_fwd": (None, [_I32, _P, _P, _P, _P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _I32,
                                             _U64, _P, _I64, _I32, _I32, _I32, _I32, _P, _I32, _I32, _I32, _I32,
                                             _P, _I32, _I32, _F, _P, _I32]),
-            "oracle_projector_bwd": (None, [_I32, _P, _P, _P, _P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _I32,
+            "oracle_projector_bwd": (None, [_I32, _P, _P, _P, _P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _I32,
                                             _U64, _P, _I64, _I32, _I32, _I32, _I32, _P, _I64, _I32, _I32, _I32,
                                             _I32, _P, _I32, _I32, _F, _P, _P, _P, _I32]),
             "oracle_max_threads": (_I32, []),
@@ -246,9 +246,12 @@ def _proj_common(prec, tree, prefix, origin, direction, segs, grid, mlp, jitter,
                 sr=_c(segs["seg_ray"], np.int32), sc=_c(segs["seg_count"], np.int32),
                 st0=_c(segs["seg_tmin"], np.float32), st1=_c(segs["seg_tmax"], np.float32),
                 params=_c(mlp["params"], np.float32))
+    # optional proposal CDF [K, 8] (rows with cdf[7] == 0 sample uniformly)
+    keep["cdf"] = _c(segs["seg_cdf"], np.float32) if segs.get("seg_cdf") is not None else None
     args = [prec, _p(keep["pmin"]), _p(keep["pmax"]), _p(keep["prefix"]), _p(o), _p(d), o.shape[0],
             _p(keep["seg_off"]), keep["sn"].shape[0], _p(keep["sn"]), _p(keep["sr"]), _p(keep["sc"]),
-            _p(keep["st0"]), _p(keep["st1"]), int(bool(jitter)), seed, _p(g), N, C, D, H, W, _p(strides)]
+            _p(keep["st0"]), _p(keep["st1"]), _p(keep["cdf"]) if keep["cdf"] is not None else None,
+            int(bool(jitter)), seed, _p(g), N, C, D, H, W, _p(strides)]
     return keep, args
 
 

@@ -105,6 +105,10 @@ def test_fcblock_matches_torch_linear(H):
 @pytest.mark.parametrize("fused", [True, False])
 def test_compute_image_and_backward_vs_oracle(H, oracle, fused):
     print("case: GeometryExEx.ComputeImage fused =", fused, flush=True)
+    # The jitter seed comes from torch's generator: fix it. With relu hidden units a sample whose pre-activation lies
+    # within rounding of 0 can take the other side of the kink in the oracle, which moves that sample's whole
+    # gradient contribution; fixed seeds keep the comparison reproducible.
+    torch.manual_seed(0)
     sc, tree, p, geo = make_geometry(H, W.C1)
     p.train_params.use_fused_projector = fused
     o, d = W.batch_rays(W.C1, 5000, seed=4)
@@ -122,7 +126,8 @@ def test_compute_image_and_backward_vs_oracle(H, oracle, fused):
     mlp = dict(sc["mlp"], params=sc["params"])
     ref = oracle.projector_fwd(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, jitter=True, seed=seg.seed)
     assert rel(img.detach().cpu().numpy()[0], ref) < 1e-5
-    gproj = 2 * (ref - tgt.cpu().numpy()) / o.shape[0]
+    # the GPU's gradient is the gradient of the GPU's loss: dL/dproj from the GPU projection
+    gproj = 2 * (img.detach().cpu().numpy()[0].astype(np.float64) - tgt.cpu().numpy()) / o.shape[0]
     rgg, rgp = oracle.projector_bwd(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, gproj, jitter=True,
                                     seed=seg.seed)
     gg = geo.grid.grad.cpu().numpy()
@@ -139,6 +144,7 @@ def test_compute_image_and_backward_vs_oracle(H, oracle, fused):
 @pytest.mark.parametrize("precision", ["bf16", "fp32_valu"])
 def test_compute_image_precision_forms(H, oracle, precision):
     print("case: GeometryExEx.ComputeImage precision =", precision, flush=True)
+    torch.manual_seed(1)
     sc, tree, p, geo = make_geometry(H, W.C1)
     p.train_params.use_fused_projector = True
     p.train_params.projector_precision = precision
