@@ -37,11 +37,28 @@ def parse():
     ap.add_argument("--cpu-rays", type=int, default=2048, help="rays in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (the reference kernel for scaling HBM "
+                         "byte counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}), flush=True)
+    return 0
 
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        return copy_calibration(a.copy_calibration)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it", "n_gpus": a.gpus}))
         return 2

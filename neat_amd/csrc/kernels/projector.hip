@@ -169,12 +169,12 @@ struct Corners
 
 __device__ __forceinline__ int clamp_g(int v, int G) { return min(G - 1, max(v, 0)); }
 
-__device__ __forceinline__ Corners corners_cl(const float uv[3], int G, int C)
+// Continuous grid coordinate of a local coordinate (align_corners, IndirectGridSample3D.cu:448-455).
+__device__ __forceinline__ float grid_coord(float u, int G) { return ((u + 1) / 2) * (float)(G - 1); }
+
+__device__ __forceinline__ Corners corners_ix(float ix, float iy, float iz, int G, int C)
 {
     Corners c;
-    float ix = ((uv[0] + 1) / 2) * (float)(G - 1);
-    float iy = ((uv[1] + 1) / 2) * (float)(G - 1);
-    float iz = ((uv[2] + 1) / 2) * (float)(G - 1);
     int x0 = (int)floorf(ix), y0 = (int)floorf(iy), z0 = (int)floorf(iz);
     int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
     c.w[0] = ((float)x1 - ix) * ((float)y1 - iy) * ((float)z1 - iz);
@@ -198,6 +198,11 @@ __device__ __forceinline__ Corners corners_cl(const float uv[3], int G, int C)
     c.off[6] = cz1 * zs + cy1 * ys + cx0 * C;
     c.off[7] = cz1 * zs + cy1 * ys + cx1 * C;
     return c;
+}
+
+__device__ __forceinline__ Corners corners_cl(const float uv[3], int G, int C)
+{
+    return corners_ix(grid_coord(uv[0], G), grid_coord(uv[1], G), grid_coord(uv[2], G), G, C);
 }
 
 template <int C>
@@ -580,8 +585,20 @@ constexpr int kScatterThreads = 1024;
 // vertices). The chunk's samples are cut into 16 x 64 / C contiguous streams, one per lane group, so the samples
 // of one instruction lie far apart along the node-major order (different segments / cells: few same-address
 // collisions); a stream walks its samples in order and reloads the segment state only at segment boundaries.
+// 2^F with F = 45 - exponent(max |dL/dfeature|): every contribution is below 2^45 in fixed point and a vertex's sum
+// of fewer than 2^18 of them stays inside int64; quantisation is 2^-46 of the largest value per contribution.
+__device__ __forceinline__ double fixed_scale(unsigned int maxbits)
+{
+    const float m = __uint_as_float(maxbits);
+    if (!(m > 0.f) || !isfinite(m)) return 1.0;
+    int e;
+    frexpf(m, &e);
+    return ldexp(1.0, 45 - e);
+}
+
 template <int C>
 __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p, const float* dfeat,
+                                                              const float4* sgc, const unsigned int* dmax,
                                                               float* grad_grid, float* partial_grids)
 {
     extern __shared__ float sg[];
@@ -603,6 +620,67 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
     const int64_t L    = (ns + U - 1) / U;
     int64_t s          = u * L;
     const int64_t send = min(ns, s + L);
+    if constexpr (C == 8)
+    {
+      if (sgc)
+      {
+        // The backward kept every sample's grid coordinate. Channel half blockIdx.y of the node grid is accumulated
+        // in 64-bit fixed point (value x 2^F, F from the batch's largest |dL/dfeature|): integer adds are exact and
+        // order-independent, so the result is bit-reproducible, and LDS integer atomics run several times faster
+        // than float ones. Lanes are (corner, channel) pairs of 2 samples, so one instruction's 64 adds hit 64
+        // distinct addresses; each wave streams a contiguous run of the chunk's samples, one sample ahead.
+        unsigned long long* sq = reinterpret_cast<unsigned long long*>(sg);
+        const int sub = lane >> 5, kk = (lane >> 2) & 7, c4 = lane & 3;
+        const int64_t U2 = (int64_t)(blockDim.x >> 6) * 2;
+        const int64_t L2 = (ns + U2 - 1) / U2;
+        int64_t s2       = ((int64_t)wave * 2 + sub) * L2;
+        const int64_t e2 = min(ns, s2 + L2);
+        const double scale = fixed_scale(*dmax);
+        const int ch0      = 4 * (int)blockIdx.y;
+        if (s2 < e2)
+        {
+            float4 gc = sgc[base + s2];
+            float dv = dfeat[(base + s2) * C + ch0 + c4];
+            for (; s2 < e2; ++s2)
+            {
+                float4 cn = gc;
+                float dn  = dv;
+                if (s2 + 1 < e2)
+                {
+                    cn = sgc[base + s2 + 1];
+                    dn = dfeat[(base + s2 + 1) * C + ch0 + c4];
+                }
+                // corner kk of corners_ix, the same operations
+                const int x0 = (int)floorf(gc.x), y0 = (int)floorf(gc.y), z0 = (int)floorf(gc.z);
+                const int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+                const float fx = (kk & 1) ? (gc.x - (float)x0) : ((float)x1 - gc.x);
+                const float fy = (kk & 2) ? (gc.y - (float)y0) : ((float)y1 - gc.y);
+                const float fz = (kk & 4) ? (gc.z - (float)z0) : ((float)z1 - gc.z);
+                const float wk = fx * fy * fz;
+                const int cx = clamp_g((kk & 1) ? x1 : x0, g.G), cy = clamp_g((kk & 2) ? y1 : y0, g.G),
+                          cz = clamp_g((kk & 4) ? z1 : z0, g.G);
+                const long long q = __double2ll_rn((double)(wk * dv) * scale);
+                atomicAdd(sq + ((cz * g.G + cy) * g.G + cx) * 4 + c4, (unsigned long long)q);
+                gc = cn;
+                dv = dn;
+            }
+        }
+        __syncthreads();
+        const bool split = p.chunk_off2[slot + 1] - p.chunk_off2[slot] > 1;
+        float* dst       = split ? partial_grids + c * gsize : grad_grid + (int64_t)slot * gsize;
+        const double inv = 1.0 / scale;
+        for (int32_t v = threadIdx.x; v < g.G * g.G * g.G; v += blockDim.x)
+        {
+            float4 o;
+            o.x = (float)((double)(long long)sq[4 * v + 0] * inv);
+            o.y = (float)((double)(long long)sq[4 * v + 1] * inv);
+            o.z = (float)((double)(long long)sq[4 * v + 2] * inv);
+            o.w = (float)((double)(long long)sq[4 * v + 3] * inv);
+            *reinterpret_cast<float4*>(dst + (int64_t)v * C + ch0) = o;
+        }
+        return;
+      }
+    }
     if (s < send)
     {
         const int32_t node = g.seg_node[p.perm[q0]];
@@ -874,7 +952,7 @@ constexpr int kXOnes1 = 8, kXDz = 9, kXOnes2 = 10, kXRows = 11;
 template <typename P, int H, int ACT>
 __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const float* __restrict__ params,
                                                   const void* grid_lowp, const float* grad_proj, float* dfeat,
-                                                  float* partials, int32_t P_total)
+                                                  float4* sgc, unsigned int* dmax, float* partials, int32_t P_total)
 {
     constexpr int MT = H / 32, S = steps_per_tile<P>(), C = 8;
     constexpr int KSTEPS = 32 / (2 * P::KS);  // sample steps of a weight-gradient product (FP32 16, BF16 2)
@@ -895,6 +973,7 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
         for (int b = 0; b < MT; ++b) dW2[a][b] = f32x16{};
     }
     float b3 = 0.f;
+    float amax = 0.f;  // largest |dL/dfeature| this lane wrote
     // the constant rows of the X tile
     if (hg == 0)
     {
@@ -930,9 +1009,12 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
                 {
                     const int32_t t = cx.sq[i];
                     Sample smp      = ctx_sample(g, cx, t, w0 + i - cx.beg[t], mn, mx);
-                    Corners cr      = corners_cl(smp.uv, g.G, C);
+                    const float ix = grid_coord(smp.uv[0], g.G), iy = grid_coord(smp.uv[1], g.G),
+                                iz = grid_coord(smp.uv[2], g.G);
+                    Corners cr = corners_ix(ix, iy, iz, g.G, C);
                     gather_half<P::kBF16>(ngrid, cr, hg, x4);
                     gy = cx.gproj[t] * smp.w;
+                    if (hg == 0) sgc[base + w0 + i] = make_float4(ix, iy, iz, 0.f);
                 }
                 Fwd<MT> f;
                 tile_forward_mfma<P, H, ACT>(w, x4, g.beta, f);
@@ -1017,8 +1099,11 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
                             dx = P::mma(w.w1t[ms][st][l], P::from_acc(dZ1[ms], st, [](float v, int) { return v; }), dx);
                     }
                     if (valid)
+                    {
                         *reinterpret_cast<float4*>(dfeat + (base + w0 + i) * C + 4 * hg) =
                             make_float4(dx[0], dx[1], dx[2], dx[3]);
+                        amax = fmaxf(amax, fmaxf(fmaxf(fabsf(dx[0]), fabsf(dx[1])), fmaxf(fabsf(dx[2]), fabsf(dx[3]))));
+                    }
                 }
                 // E += dZ1 [x; 1]^T: the first-layer weight gradient (columns 0..7) and bias gradient (column 8)
 #pragma unroll
@@ -1069,6 +1154,9 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
 #pragma unroll
     for (int off = 16; off >= 1; off >>= 1) b3 += __shfl_xor(b3, off);
     if (l == 0) part[ob2] = b3;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off));
+    if (l == 0) atomicMax(dmax, __float_as_uint(amax));  // non-negative floats order like their bit patterns
 }
 
 // ---- workspace layout ---------------------------------------------------------------------------------------------------
@@ -1132,6 +1220,8 @@ Layout proj_layout(const ProjSizes& z)
     L.add(4 * z.K);                       // 11 seg_partial (forward)
     L.add(4 * (size_t)z.P);               // 12 transposed decoder params
     L.add(4 * (size_t)z.S * z.C);         // 13 dfeat
+    L.add(16 * (size_t)z.S);              // 14 grid coordinates (ix, iy, iz, -) per sample, matrix-core backward
+    L.add(256);                           // 15 max |dL/dfeature| (float bits), matrix-core backward
     return L;
 }
 
@@ -1284,15 +1374,16 @@ int launch_fwd_mfma(hipStream_t s, const Args& a, const Plan& p, float* seg_part
 
 // Scatter of dL/dfeature into the node gradient grids (pass 2), shared by both backward variants.
 template <int C>
-int launch_scatter(hipStream_t s, const Args& a, const Plan& p, const float* dfeat, float* grid_partials,
-                   float* grad_grid)
+int launch_scatter(hipStream_t s, const Args& a, const Plan& p, const float* dfeat, const float4* sgc,
+                   const unsigned int* dmax, float* grid_partials, float* grad_grid)
 {
     const int32_t gsize = a.g.G * a.g.G * a.g.G * C;
     const size_t smem   = (size_t)gsize * sizeof(float);
     NEAT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_proj_bw2<C>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-    k_proj_bw2<C><<<(unsigned)std::max<int64_t>(1, a.z.max2), kScatterThreads, smem, s>>>(a.g, p, dfeat, grad_grid,
-                                                                                           grid_partials);
+    // the fixed-point form holds half the channels per workgroup as int64 (the same bytes as all of them in FP32)
+    const dim3 grid((unsigned)std::max<int64_t>(1, a.z.max2), sgc ? C / 4 : 1);
+    k_proj_bw2<C><<<grid, kScatterThreads, smem, s>>>(a.g, p, dfeat, sgc, dmax, grad_grid, grid_partials);
     NEAT_LAUNCHED();
     dim3 rg(grid_for(gsize, 256), (unsigned)a.g.N);
     k_proj_bw2_reduce<<<rg, 256, 0, s>>>(a.g, p, gsize, grid_partials, grad_grid);
@@ -1313,25 +1404,27 @@ int launch_bw_valu(hipStream_t s, const Args& a, const Plan& p, const float* gra
     NEAT_LAUNCHED();
     k_dec_reduce_partials<<<grid_for(P, 256), 256, 0, s>>>(partials, nwg, P, grad_params);
     NEAT_LAUNCHED();
-    return launch_scatter<C>(s, a, p, dfeat, grid_partials, grad_grid);
+    return launch_scatter<C>(s, a, p, dfeat, nullptr, nullptr, grid_partials, grad_grid);
 }
 
 template <typename P, int H>
-int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const float* grad_proj, float* dfeat, float* partials,
-                   float* grid_partials, float* grad_grid, float* grad_params)
+int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const float* grad_proj, float* dfeat, float4* sgc,
+                   unsigned int* dmax, float* partials, float* grid_partials, float* grad_grid, float* grad_params)
 {
     const int nwg  = mfma_bwd_workgroups(a.z);
     const int Ptot = a.m.nparams();
+    k_fill_f32<<<1, 64, 0, s>>>(reinterpret_cast<float*>(dmax), 1, 0.f);
+    NEAT_LAUNCHED();
     if (a.act == NEAT_ACT_RELU)
-        k_mfma_bwd<P, H, NEAT_ACT_RELU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, grad_proj, dfeat, partials,
-                                                             Ptot);
+        k_mfma_bwd<P, H, NEAT_ACT_RELU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, grad_proj, dfeat, sgc,
+                                                             dmax, partials, Ptot);
     else
-        k_mfma_bwd<P, H, NEAT_ACT_SILU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, grad_proj, dfeat, partials,
-                                                             Ptot);
+        k_mfma_bwd<P, H, NEAT_ACT_SILU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, grad_proj, dfeat, sgc,
+                                                             dmax, partials, Ptot);
     NEAT_LAUNCHED();
     k_dec_reduce_partials<<<grid_for(Ptot, 256), 256, 0, s>>>(partials, nwg * 4, Ptot, grad_params);
     NEAT_LAUNCHED();
-    return launch_scatter<8>(s, a, p, dfeat, grid_partials, grad_grid);
+    return launch_scatter<8>(s, a, p, dfeat, sgc, dmax, grid_partials, grad_grid);
 }
 
 int dispatch_fwd(hipStream_t s, const Args& a, const Plan& p, float* seg_partial)
@@ -1350,16 +1443,16 @@ int dispatch_fwd(hipStream_t s, const Args& a, const Plan& p, float* seg_partial
 }
 
 int dispatch_bwd(hipStream_t s, const Args& a, const Plan& p, const float* gproj, float* wt, float* dfeat,
-                 float* partials, float* gpart, float* ggrid, float* gparams)
+                 float4* sgc, unsigned int* dmax, float* partials, float* gpart, float* ggrid, float* gparams)
 {
     const int C = a.m.in, H = a.m.hidden;
     if (a.mfma)
     {
         if (a.precision == NEAT_PREC_BF16)
-            return H == 64 ? launch_bw_mfma<PrecBF16, 64>(s, a, p, gproj, dfeat, partials, gpart, ggrid, gparams)
-                           : launch_bw_mfma<PrecBF16, 32>(s, a, p, gproj, dfeat, partials, gpart, ggrid, gparams);
-        return H == 64 ? launch_bw_mfma<PrecF32, 64>(s, a, p, gproj, dfeat, partials, gpart, ggrid, gparams)
-                       : launch_bw_mfma<PrecF32, 32>(s, a, p, gproj, dfeat, partials, gpart, ggrid, gparams);
+            return H == 64 ? launch_bw_mfma<PrecBF16, 64>(s, a, p, gproj, dfeat, sgc, dmax, partials, gpart, ggrid, gparams)
+                           : launch_bw_mfma<PrecBF16, 32>(s, a, p, gproj, dfeat, sgc, dmax, partials, gpart, ggrid, gparams);
+        return H == 64 ? launch_bw_mfma<PrecF32, 64>(s, a, p, gproj, dfeat, sgc, dmax, partials, gpart, ggrid, gparams)
+                       : launch_bw_mfma<PrecF32, 32>(s, a, p, gproj, dfeat, sgc, dmax, partials, gpart, ggrid, gparams);
     }
     if (C == 8)
         return H == 64 ? launch_bw_valu<8, 64>(s, a, p, gproj, wt, dfeat, partials, gpart, ggrid, gparams)
@@ -1528,6 +1621,8 @@ extern "C"
             NEAT_LAUNCHED();
             return NEAT_OK;
         }
-        return dispatch_bwd(s, a, p, grad_proj, wt, dfeat, partials, gpartials, grad_grid, grad_params);
+        float4* sgc      = reinterpret_cast<float4*>(region(ws, z, 14));
+        unsigned int* dm = reinterpret_cast<unsigned int*>(region(ws, z, 15));
+        return dispatch_bwd(s, a, p, grad_proj, wt, dfeat, sgc, dm, partials, gpartials, grad_grid, grad_params);
     }
 }

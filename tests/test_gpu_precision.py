@@ -120,9 +120,9 @@ def test_compensated_ray_sums_and_determinism(oracle, precision):
     gproj = np.ones(o.shape[0], np.float32)
     p1, g1, q1 = fused(ds, to, td, segs, False, 0, torch.from_numpy(gproj).to(DEV), precision, compensated=True)
     p2, g2, q2 = fused(ds, to, td, segs, False, 0, torch.from_numpy(gproj).to(DEV), precision, compensated=True)
-    # the forward is bit-reproducible; the grid gradient's LDS scatter adds in hardware order
-    assert np.array_equal(p1, p2) and np.array_equal(q1, q2)
-    assert rel(g1, g2) <= 1e-6
+    # forward, decoder gradient (fixed-order partial sums) and grid gradient (integer fixed-point scatter) are all
+    # bit-reproducible
+    assert np.array_equal(p1, p2) and np.array_equal(q1, q2) and np.array_equal(g1, g2)
     ref = oracle_both(oracle, sc, o, d, segs, False, 0, gproj)
     e = rel(p1, ref[1][0])
     print(precision, "compensated projection error vs fp64 oracle", e)
