@@ -34,7 +34,7 @@ def parse():
                     help="fused path arithmetic: BF16 on the matrix cores (default), FP32 on the matrix cores, or "
                          "FP32 on the vector ALUs")
     ap.add_argument("--batches", type=int, default=4, help="distinct resident ray batches cycled by the steps")
-    ap.add_argument("--cpu-rays", type=int, default=2048, help="rays in the CPU-baseline sample")
+    ap.add_argument("--cpu-rays", type=int, default=12288, help="rays in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -79,9 +79,11 @@ def main():
 
     # the BF16 grid copy is refreshed once per optimizer step in training; here the grid is fixed, so once
     grid_bf16 = ops.grid_to_bf16(ds.grid) if (a.path == "fused" and a.precision == "bf16") else None
-    stage_names = (["sample", "plan", "proj_fwd", "loss", "proj_bwd"] if a.path == "fused" else
-                   ["sample", "gs_fwd", "dec_fwd", "integrate", "dec_bwd", "gs_bwd"])
+    stage_names = (["sample", "plan", "proj_fwd", "loss", "bwd_decoder", "bwd_params", "bwd_grid"]
+                   if a.path == "fused" else ["sample", "gs_fwd", "dec_fwd", "integrate", "dec_bwd", "gs_bwd"])
     events = {k: [] for k in stage_names}
+    gg_buf = torch.empty_like(ds.grid)
+    gp_buf = torch.empty_like(ds.params)
 
     def fused_step(i, timed):
         o, d, tgt = batches[i % len(batches)]
@@ -100,9 +102,11 @@ def main():
         rec("loss", 0)
         loss, gproj = ops.mse_fp64(proj, tgt)
         rec("loss", 1)
-        rec("proj_bwd", 0)
-        ggrid, gparams = pj.backward(gproj)
-        rec("proj_bwd", 1)
+        # the backward's three stages (decoder, parameter gradient, grid gradient), timed one by one
+        for st, name in ((1, "bwd_decoder"), (2, "bwd_params"), (3, "bwd_grid")):
+            rec(name, 0)
+            pj.backward_stage(st, gproj, gg_buf, gp_buf)
+            rec(name, 1)
         if timed:
             for k in stage_names:
                 events[k].append(ev[k])
@@ -158,27 +162,37 @@ def main():
     stage_ms = {k: float(np.mean([s.elapsed_time(e) for s, e in events[k]])) for k in stage_names}
     dominant = max(stage_ms, key=stage_ms.get)
     C, H = cfg.channels, cfg.hidden
-    fwd_flop = 2 * (C * H + H * H + H * 1)  # per sample, 2 FLOP per MAC
+    fwd_flop = 2 * (C * H + H * H + H * 1)  # decoder forward per sample, 2 FLOP per MAC
     mm_peak = BF16_PEAK_TFLOPS if (a.path == "fused" and a.precision == "bf16") else FP32_PEAK_TFLOPS
-    # algorithmic work per launch of each stage (per-sample figure x samples of the batch)
+    mm = "fp32_valu" != a.precision
+    prec_tag = {"bf16": "PrecBF16", "fp32": "PrecF32"}.get(a.precision)
+    act = 1 if cfg.act == "relu" else 2
+    grid_bytes = ds.grid.numel() * 4
+    # algorithmic work per launch of each stage's dominant kernel (per-unit figure x the units of one launch):
+    # decoder FLOPs per sample (backward: 2x the forward, no recomputation counted); the scatter moves 48 B per
+    # sample (grid coordinate 16 B + dL/dfeature 32 B) and writes the gradient grid once
     work = {
-        "proj_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", mm_peak),
-        "proj_bwd": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", mm_peak),
-        "plan": ("hbm", 16 * mean_samples / 27, "GB/s", HBM_PEAK_GBS),
-        "loss": ("hbm", 12 * cfg.batch, "GB/s", HBM_PEAK_GBS),
-        "dec_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS),
-        "dec_bwd": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS),
-        "gs_fwd": ("hbm", (12 + 8 + 4 * C) * mean_samples, "GB/s", HBM_PEAK_GBS),
-        "gs_bwd": ("hbm", (12 + 8 + 4 * C) * mean_samples + 2 * ds.grid.numel() * 4, "GB/s", HBM_PEAK_GBS),
-        "sample": ("hbm", 44 * mean_samples + 32 * cfg.batch, "GB/s", HBM_PEAK_GBS),
-        "integrate": ("hbm", (4 + 4 + 8) * mean_samples * 2 + 16 * cfg.batch, "GB/s", HBM_PEAK_GBS),
+        "proj_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", mm_peak,
+                     f"k_mfma_fwd<{prec_tag},{H},{act}>" if mm and prec_tag else f"k_proj_fwd<{C},{H},{act}>"),
+        "bwd_decoder": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", mm_peak,
+                        f"k_mfma_bwd<{prec_tag},{H},{act}>" if mm and prec_tag else f"k_proj_bw1<{C},{H}>"),
+        "bwd_grid": ("hbm", 48 * mean_samples + grid_bytes, "GB/s", HBM_PEAK_GBS, f"k_proj_bw2<{C}>"),
+        "bwd_params": ("hbm", 4.0 * 1024 * ds.params.numel(), "GB/s", HBM_PEAK_GBS, "k_dec_reduce_partials"),
+        "plan": ("hbm", 16 * mean_samples / 27, "GB/s", HBM_PEAK_GBS, "plan"),
+        "loss": ("hbm", 12 * cfg.batch, "GB/s", HBM_PEAK_GBS, "k_mse"),
+        "dec_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS, "k_dec_fwd"),
+        "dec_bwd": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS, "k_dec_bwd"),
+        "gs_fwd": ("hbm", (12 + 8 + 4 * C) * mean_samples, "GB/s", HBM_PEAK_GBS, "k_gs3d_fwd"),
+        "gs_bwd": ("hbm", (12 + 8 + 4 * C) * mean_samples + 2 * grid_bytes, "GB/s", HBM_PEAK_GBS, "k_gs3d_bwd"),
+        "sample": ("hbm", 44 * mean_samples / 27 + 32 * cfg.batch, "GB/s", HBM_PEAK_GBS, "k_ray_segments"),
+        "integrate": ("hbm", (4 + 4 + 8) * mean_samples * 2 + 16 * cfg.batch, "GB/s", HBM_PEAK_GBS, "k_integrate"),
     }
-    bound, amount, unit, peak = work[dominant]
+    bound, amount, unit, peak, kname = work[dominant]
     secs = stage_ms[dominant] / 1e3
     achieved = amount / secs / (1e12 if unit == "TFLOP/s" else 1e9)
     roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                "frac": round(achieved / peak, 5), "traffic": None, "kernel": dominant,
-                "kernel_ms": round(stage_ms[dominant], 4)}
+                "frac": round(achieved / peak, 5), "traffic": measured_traffic(kname), "kernel": kname,
+                "stage": dominant, "kernel_ms": round(stage_ms[dominant], 4)}
 
     cpu = None
     if not a.no_cpu:
@@ -208,6 +222,21 @@ def main():
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary (FETCH_SIZE, WRITE_SIZE in KiB
+    per dispatch, each scaled by the reading of a plain copy of known size), or None when not profiled."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return round(1024 * (k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"]))
 
 
 def cpu_baseline(scene, cfg, num_rays, seed):

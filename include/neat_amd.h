@@ -263,6 +263,14 @@ int neat_projector_fwd(neat_stream_t stream, const neat_projector_args* args, in
 int neat_projector_bwd(neat_stream_t stream, const neat_projector_args* args, const float* grad_proj,
                        float* grad_grid, float* grad_params, void* workspace, size_t workspace_bytes);
 
+/* The backward in its three stages, for timing them and for overlapping them with other work: 1 the decoder backward
+ * (dL/dfeature per sample and per-workgroup parameter partials, into the workspace), 2 the parameter gradient (the
+ * partials summed in a fixed order, into grad_params), 3 the grid gradient (the scatter, into grad_grid). Stages 1, 2,
+ * 3 in order on one stream are neat_projector_bwd; only the buffer of a stage's output needs to be non-NULL. */
+int neat_projector_bwd_stage(neat_stream_t stream, const neat_projector_args* args, int32_t stage,
+                             const float* grad_proj, float* grad_grid, float* grad_params, void* workspace,
+                             size_t workspace_bytes);
+
 /* Per-sample decoded values softplus(decoder(trilinear(grid))) (unweighted) of the planned segment list, in the
  * plan's node-major sample order (values [num_samples]), and seg_value_offset [num_segments] = the first sample of each
  * canonical segment in that order. Matrix-core kernels only (8 channels, NEAT_PREC_FP32 or NEAT_PREC_BF16). */

@@ -86,9 +86,21 @@ __host__ __device__ __forceinline__ uint64_t mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
+// the part of the hash that depends on the seed and the ray only
+__host__ __device__ __forceinline__ uint64_t jitter_ray_hash(uint64_t seed, int64_t ray)
+{
+    return mix64(seed ^ ((uint64_t)ray * 0x9E3779B97F4A7C15ull));
+}
+
+__host__ __device__ __forceinline__ float jitter_a_h(uint64_t hray, int32_t rank, int32_t j)
+{
+    const uint64_t h = mix64(hray ^ (((uint64_t)(uint32_t)rank << 32) | (uint64_t)(uint32_t)j));
+    return (float)(h >> 40) * (1.0f / 16777216.0f);
+}
+
 __host__ __device__ __forceinline__ float jitter_a(uint64_t seed, int64_t ray, int32_t rank, int32_t j)
 {
-    uint64_t h = mix64(seed ^ ((uint64_t)ray * 0x9E3779B97F4A7C15ull));
+    uint64_t h = jitter_ray_hash(seed, ray);
     h          = mix64(h ^ (((uint64_t)(uint32_t)rank << 32) | (uint64_t)(uint32_t)j));
     return (float)(h >> 40) * (1.0f / 16777216.0f);
 }

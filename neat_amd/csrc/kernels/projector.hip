@@ -639,31 +639,61 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
         const int ch0      = 4 * (int)blockIdx.y;
         if (s2 < e2)
         {
-            float4 gc = sgc[base + s2];
-            float dv = dfeat[(base + s2) * C + ch0 + c4];
-            for (; s2 < e2; ++s2)
-            {
-                float4 cn = gc;
-                float dn  = dv;
-                if (s2 + 1 < e2)
+            // Consecutive samples of a stream usually fall in the same cell: a lane keeps its (corner, channel) sum
+            // in a register while the address stays the same and adds it to LDS when the address changes (exact:
+            // the sum is an integer either way). Loads go in blocks of kSB samples, the next block in flight while
+            // the current one is accumulated.
+            constexpr int kSB = 8;
+            int addr      = -1;
+            long long acc = 0;
+            auto load = [&](int64_t s0, float4 (&gb)[kSB], float (&db)[kSB]) {
+#pragma unroll
+                for (int b = 0; b < kSB; ++b)
                 {
-                    cn = sgc[base + s2 + 1];
-                    dn = dfeat[(base + s2 + 1) * C + ch0 + c4];
+                    if (s0 + b < e2)
+                    {
+                        gb[b] = sgc[base + s0 + b];
+                        db[b] = dfeat[(base + s0 + b) * C + ch0 + c4];
+                    }
                 }
-                // corner kk of corners_ix, the same operations
-                const int x0 = (int)floorf(gc.x), y0 = (int)floorf(gc.y), z0 = (int)floorf(gc.z);
-                const int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
-                const float fx = (kk & 1) ? (gc.x - (float)x0) : ((float)x1 - gc.x);
-                const float fy = (kk & 2) ? (gc.y - (float)y0) : ((float)y1 - gc.y);
-                const float fz = (kk & 4) ? (gc.z - (float)z0) : ((float)z1 - gc.z);
-                const float wk = fx * fy * fz;
-                const int cx = clamp_g((kk & 1) ? x1 : x0, g.G), cy = clamp_g((kk & 2) ? y1 : y0, g.G),
-                          cz = clamp_g((kk & 4) ? z1 : z0, g.G);
-                const long long q = __double2ll_rn((double)(wk * dv) * scale);
-                atomicAdd(sq + ((cz * g.G + cy) * g.G + cx) * 4 + c4, (unsigned long long)q);
-                gc = cn;
-                dv = dn;
+            };
+            auto accumulate = [&](int64_t s0, const float4 (&gb)[kSB], const float (&db)[kSB]) {
+#pragma unroll
+                for (int b = 0; b < kSB; ++b)
+                {
+                    if (s0 + b >= e2) break;
+                    const float4 gc = gb[b];
+                    // corner kk of corners_ix, the same operations
+                    const int x0 = (int)floorf(gc.x), y0 = (int)floorf(gc.y), z0 = (int)floorf(gc.z);
+                    const int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+                    const float fx = (kk & 1) ? (gc.x - (float)x0) : ((float)x1 - gc.x);
+                    const float fy = (kk & 2) ? (gc.y - (float)y0) : ((float)y1 - gc.y);
+                    const float fz = (kk & 4) ? (gc.z - (float)z0) : ((float)z1 - gc.z);
+                    const float wk = fx * fy * fz;
+                    const int cx = clamp_g((kk & 1) ? x1 : x0, g.G), cy = clamp_g((kk & 2) ? y1 : y0, g.G),
+                              cz = clamp_g((kk & 4) ? z1 : z0, g.G);
+                    const long long q = __double2ll_rn((double)(wk * db[b]) * scale);
+                    const int a       = ((cz * g.G + cy) * g.G + cx) * 4 + c4;
+                    if (a != addr)
+                    {
+                        if (addr >= 0) atomicAdd(sq + addr, (unsigned long long)acc);
+                        addr = a;
+                        acc  = 0;
+                    }
+                    acc += q;
+                }
+            };
+            float4 ga[kSB], gb[kSB];
+            float da[kSB], db[kSB];
+            load(s2, ga, da);
+            for (int64_t s0 = s2; s0 < e2; s0 += 2 * kSB)
+            {
+                load(s0 + kSB, gb, db);
+                accumulate(s0, ga, da);
+                load(s0 + 2 * kSB, ga, da);
+                accumulate(s0 + kSB, gb, db);
             }
+            if (addr >= 0) atomicAdd(sq + addr, (unsigned long long)acc);
         }
         __syncthreads();
         const bool split = p.chunk_off2[slot + 1] - p.chunk_off2[slot] > 1;
@@ -791,6 +821,7 @@ struct ChunkCtx
     float gproj[kSegCap];  // dL/dprojection of the segment's ray (backward)
     float o[3][kSegCap], d[3][kSegCap];
     float cdf[8][kSegCap];  // proposal CDF rows (cdf[7][t] == 0: uniform)
+    uint32_t hray[2][kSegCap];  // jitter_ray_hash of the segment's ray (low, high word)
     uint16_t sq[kFwdChunk];  // window-relative sample -> chunk-relative segment
 };
 
@@ -815,6 +846,9 @@ __device__ void load_chunk_ctx(const ProjGeom& g, const Plan& p, int32_t q0, int
         c.tmax[t]  = tmax;
         c.step[t]  = (tmax - tmin) / (float)n;
         c.gproj[t] = grad_proj ? grad_proj[ray] : 0.f;
+        const uint64_t hr = jitter_ray_hash(g.seed, ray);
+        c.hray[0][t]      = (uint32_t)hr;
+        c.hray[1][t]      = (uint32_t)(hr >> 32);
         float cdf[8];
         const bool prop = load_cdf(g, k, cdf);
 #pragma unroll
@@ -846,11 +880,28 @@ __device__ __forceinline__ Sample ctx_sample(const ProjGeom& g, const ChunkCtx& 
     const int64_t ray  = c.ray[t];
     const int32_t rank = c.rank[t], n = c.n[t];
     const float tmin = c.tmin[t], tmax = c.tmax[t], step = c.step[t];
-    float cdf[8];
-#pragma unroll
-    for (int b = 0; b < 8; ++b) cdf[b] = c.cdf[b][t];
+    const uint64_t hr = ((uint64_t)c.hray[1][t] << 32) | c.hray[0][t];
+    auto a_of         = [&](int32_t jj) { return g.jitter ? jitter_a_h(hr, rank, jj) : 0.5f; };
     float tt;
-    sample_tw(g, cdf[7] > 0.f, cdf, tmin, tmax, step, n, j, ray, rank, tt, s.w);
+    if (c.cdf[7][t] > 0.f)
+    {
+        float cdf[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) cdf[b] = c.cdf[b][t];
+        const float t1 = proposal_cell(cdf, tmin, tmax, n, j);
+        const float t2 = proposal_cell(cdf, tmin, tmax, n, j + 1);
+        const float a  = a_of(j);
+        tt             = t1 * (1.0f - a) + t2 * a;
+        s.w            = t2 - t1;
+    }
+    else
+    {
+        // sample_t / sample_w with the ray's half of the hash taken from the context
+        tt             = sample_t(tmin, step, j, a_of(j));
+        const float lo = (j == 0) ? tmin : (sample_t(tmin, step, j - 1, a_of(j - 1)) + tt) * 0.5f;
+        const float hi = (j == n - 1) ? tmax : (sample_t(tmin, step, j + 1, a_of(j + 1)) + tt) * 0.5f;
+        s.w            = hi - lo;
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
@@ -871,10 +922,70 @@ __device__ __forceinline__ void node_bounds(const ProjGeom& g, int32_t node, flo
     }
 }
 
+// Inputs of a pair of 32-sample tiles (samples p0 .. p0 + 63 of the window): every lane computes the geometry of one
+// sample (lane l: sample p0 + l) and the lane groups swap their corners, so that each lane holds the corners of
+// sample p0 + n (tile A) and p0 + 32 + n (tile B) and gathers its 4 channels of both. v is the sample weight
+// (forward) or dL/dy = dL/dproj x weight (backward); samples past the window have zero weights, features and v.
+struct PairIn
+{
+    float xa[4], xb[4];
+    float va, vb;
+};
+
+template <bool kBF16, bool kBwd>
+__device__ __forceinline__ void fetch_pair(const ProjGeom& g, const ChunkCtx& cx, int32_t p0, int32_t w0,
+                                           int32_t wn, int64_t base, const float (&mn)[3], const float (&mx)[3],
+                                           const void* ngrid, float4* sgc, PairIn& in)
+{
+    const int l = threadIdx.x & 63, hg = l >> 5;
+    const int32_t ii = p0 + l;
+    Corners cr;
+    float v = 0.f;
+    if (ii < wn)
+    {
+        const int32_t t = cx.sq[ii];
+        Sample smp      = ctx_sample(g, cx, t, w0 + ii - cx.beg[t], mn, mx);
+        const float ix = grid_coord(smp.uv[0], g.G), iy = grid_coord(smp.uv[1], g.G), iz = grid_coord(smp.uv[2], g.G);
+        cr = corners_ix(ix, iy, iz, g.G, 8);
+        v  = kBwd ? cx.gproj[t] * smp.w : smp.w;
+        if (kBwd) sgc[base + w0 + ii] = make_float4(ix, iy, iz, 0.f);
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            cr.off[k] = 0;
+            cr.w[k]   = 0.f;
+        }
+    }
+    Corners pc;  // the other lane group's sample
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+    {
+        pc.off[k] = __shfl_xor(cr.off[k], 32);
+        pc.w[k]   = __shfl_xor(cr.w[k], 32);
+    }
+    const float pv = __shfl_xor(v, 32);
+    Corners ca, cb;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+    {
+        ca.off[k] = hg == 0 ? cr.off[k] : pc.off[k];
+        ca.w[k]   = hg == 0 ? cr.w[k] : pc.w[k];
+        cb.off[k] = hg == 1 ? cr.off[k] : pc.off[k];
+        cb.w[k]   = hg == 1 ? cr.w[k] : pc.w[k];
+    }
+    gather_half<kBF16>(ngrid, ca, hg, in.xa);
+    gather_half<kBF16>(ngrid, cb, hg, in.xb);
+    in.va = hg == 0 ? v : pv;
+    in.vb = hg == 1 ? v : pv;
+}
+
 // Forward: persistent workgroups of 4 waves over the chunks; tiles of 32 samples per wave on the matrix cores;
 // per-segment sums sequential in j as in k_proj_fwd.
 template <typename P, int H, int ACT>
-__global__ void __launch_bounds__(256) k_mfma_fwd(ProjGeom g, Plan p, const float* __restrict__ params,
+__global__ void __launch_bounds__(256, 2) k_mfma_fwd(ProjGeom g, Plan p, const float* __restrict__ params,
                                                   const void* grid_lowp, float* seg_partial, float* sample_out)
 {
     __shared__ WeightLDS<P, H> w;
@@ -903,28 +1014,23 @@ __global__ void __launch_bounds__(256) k_mfma_fwd(ProjGeom g, Plan p, const floa
             const int32_t wn = min(kFwdChunk, nsamp - w0);
             expand_window(cx, nseg, w0, wn);
             __syncthreads();
-            for (int32_t t0 = wave * 32; t0 < wn; t0 += 128)
-            {
-                const int32_t i  = t0 + n;
-                const bool valid = i < wn;
-                float x4[4]      = {0.f, 0.f, 0.f, 0.f};
-                float wgt        = 0.f;
-                if (valid)
-                {
-                    const int32_t t = cx.sq[i];
-                    Sample smp      = ctx_sample(g, cx, t, w0 + i - cx.beg[t], mn, mx);
-                    Corners cr      = corners_cl(smp.uv, g.G, 8);
-                    gather_half<P::kBF16>(ngrid, cr, hg, x4);
-                    wgt = smp.w;
-                }
+            // pairs of tiles per wave (fetch_pair)
+            auto tile = [&](int32_t i, const float (&x4)[4], float wgt) {
                 Fwd<H / 32> f;
                 tile_forward_mfma<P, H, ACT>(w, x4, g.beta, f);
-                if (valid && hg == 0)
+                if (i < wn && hg == 0)
                 {
                     const float y = act_f(NEAT_ACT_SOFTPLUS, f.z, g.beta);
                     s_contrib[i]  = y * wgt;
                     if (sample_out) sample_out[base + w0 + i] = y;
                 }
+            };
+            for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
+            {
+                PairIn cur;
+                fetch_pair<P::kBF16, false>(g, cx, p0, w0, wn, base, mn, mx, ngrid, nullptr, cur);
+                tile(p0 + n, cur.xa, cur.va);
+                if (p0 + 32 < wn) tile(p0 + 32 + n, cur.xb, cur.vb);
             }
             __syncthreads();
             for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x)
@@ -999,23 +1105,9 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
             const int32_t wn = min(kFwdChunk, nsamp - w0);
             expand_window(cx, nseg, w0, wn);
             __syncthreads();
-            for (int32_t t0 = wave * 32; t0 < wn; t0 += 128)
-            {
-                const int32_t i  = t0 + n;
+            // pairs of tiles per wave (fetch_pair)
+            auto tile = [&](int32_t i, const float (&x4)[4], float gy) {
                 const bool valid = i < wn;
-                float x4[4]      = {0.f, 0.f, 0.f, 0.f};
-                float gy         = 0.f;
-                if (valid)
-                {
-                    const int32_t t = cx.sq[i];
-                    Sample smp      = ctx_sample(g, cx, t, w0 + i - cx.beg[t], mn, mx);
-                    const float ix = grid_coord(smp.uv[0], g.G), iy = grid_coord(smp.uv[1], g.G),
-                                iz = grid_coord(smp.uv[2], g.G);
-                    Corners cr = corners_ix(ix, iy, iz, g.G, C);
-                    gather_half<P::kBF16>(ngrid, cr, hg, x4);
-                    gy = cx.gproj[t] * smp.w;
-                    if (hg == 0) sgc[base + w0 + i] = make_float4(ix, iy, iz, 0.f);
-                }
                 Fwd<MT> f;
                 tile_forward_mfma<P, H, ACT>(w, x4, g.beta, f);
                 const float dz = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, f.z, g.beta) : 0.f;
@@ -1121,6 +1213,13 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
                     for (int mt = 0; mt < MT; ++mt) E[mt] = P::mma(stage_frag<P>(sA, 32 * mt + n, st, hg, false), bx, E[mt]);
                 }
                 wave_lds_fence();
+            };
+            for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
+            {
+                PairIn cur;
+                fetch_pair<P::kBF16, true>(g, cx, p0, w0, wn, base, mn, mx, ngrid, sgc, cur);
+                tile(p0 + n, cur.xa, cur.va);
+                if (p0 + 32 < wn) tile(p0 + 32 + n, cur.xb, cur.vb);
             }
             __syncthreads();
         }
@@ -1391,40 +1490,69 @@ int launch_scatter(hipStream_t s, const Args& a, const Plan& p, const float* dfe
     return NEAT_OK;
 }
 
+// The backward's stages: 1 the decoder backward (dL/dfeature per sample, per-workgroup parameter partials),
+// 2 the parameter gradient (the partials summed in a fixed order), 3 the grid gradient (the scatter).
+struct BwdBufs
+{
+    const float* grad_proj;
+    float* wt;
+    float* dfeat;
+    float4* sgc;
+    unsigned int* dmax;
+    float* partials;
+    float* grid_partials;
+    float* grad_grid;
+    float* grad_params;
+};
+
 template <int C, int H>
-int launch_bw_valu(hipStream_t s, const Args& a, const Plan& p, const float* grad_proj, float* wt, float* dfeat,
-                   float* partials, float* grid_partials, float* grad_grid, float* grad_params)
+int launch_bw_valu(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b, int stage)
 {
     const int nwg = bw1_workgroups(a.z);
     const int P   = a.m.nparams();
-    k_transpose_params<<<grid_for(P, 256), 256, 0, s>>>(a.params, a.m, wt);
-    NEAT_LAUNCHED();
-    NEAT_HIP(hipMemsetAsync(partials, 0, (size_t)nwg * P * sizeof(float), s));
-    k_proj_bw1<C, H><<<nwg, kTile, 0, s>>>(a.g, p, a.m, wt, a.act, grad_proj, dfeat, partials);
-    NEAT_LAUNCHED();
-    k_dec_reduce_partials<<<grid_for(P, 256), 256, 0, s>>>(partials, nwg, P, grad_params);
-    NEAT_LAUNCHED();
-    return launch_scatter<C>(s, a, p, dfeat, nullptr, nullptr, grid_partials, grad_grid);
+    if (stage == 1)
+    {
+        k_transpose_params<<<grid_for(P, 256), 256, 0, s>>>(a.params, a.m, b.wt);
+        NEAT_LAUNCHED();
+        NEAT_HIP(hipMemsetAsync(b.partials, 0, (size_t)nwg * P * sizeof(float), s));
+        k_proj_bw1<C, H><<<nwg, kTile, 0, s>>>(a.g, p, a.m, b.wt, a.act, b.grad_proj, b.dfeat, b.partials);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
+    if (stage == 2)
+    {
+        k_dec_reduce_partials<<<grid_for(P, 256), 256, 0, s>>>(b.partials, nwg, P, b.grad_params);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
+    return launch_scatter<C>(s, a, p, b.dfeat, nullptr, nullptr, b.grid_partials, b.grad_grid);
 }
 
 template <typename P, int H>
-int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const float* grad_proj, float* dfeat, float4* sgc,
-                   unsigned int* dmax, float* partials, float* grid_partials, float* grad_grid, float* grad_params)
+int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b, int stage)
 {
     const int nwg  = mfma_bwd_workgroups(a.z);
     const int Ptot = a.m.nparams();
-    k_fill_f32<<<1, 64, 0, s>>>(reinterpret_cast<float*>(dmax), 1, 0.f);
-    NEAT_LAUNCHED();
-    if (a.act == NEAT_ACT_RELU)
-        k_mfma_bwd<P, H, NEAT_ACT_RELU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, grad_proj, dfeat, sgc,
-                                                             dmax, partials, Ptot);
-    else
-        k_mfma_bwd<P, H, NEAT_ACT_SILU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, grad_proj, dfeat, sgc,
-                                                             dmax, partials, Ptot);
-    NEAT_LAUNCHED();
-    k_dec_reduce_partials<<<grid_for(Ptot, 256), 256, 0, s>>>(partials, nwg * 4, Ptot, grad_params);
-    NEAT_LAUNCHED();
-    return launch_scatter<8>(s, a, p, dfeat, sgc, dmax, grid_partials, grad_grid);
+    if (stage == 1)
+    {
+        k_fill_f32<<<1, 64, 0, s>>>(reinterpret_cast<float*>(b.dmax), 1, 0.f);
+        NEAT_LAUNCHED();
+        if (a.act == NEAT_ACT_RELU)
+            k_mfma_bwd<P, H, NEAT_ACT_RELU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, b.grad_proj, b.dfeat,
+                                                                 b.sgc, b.dmax, b.partials, Ptot);
+        else
+            k_mfma_bwd<P, H, NEAT_ACT_SILU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, b.grad_proj, b.dfeat,
+                                                                 b.sgc, b.dmax, b.partials, Ptot);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
+    if (stage == 2)
+    {
+        k_dec_reduce_partials<<<grid_for(Ptot, 256), 256, 0, s>>>(b.partials, nwg * 4, Ptot, b.grad_params);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
+    return launch_scatter<8>(s, a, p, b.dfeat, b.sgc, b.dmax, b.grid_partials, b.grad_grid);
 }
 
 int dispatch_fwd(hipStream_t s, const Args& a, const Plan& p, float* seg_partial)
@@ -1442,23 +1570,18 @@ int dispatch_fwd(hipStream_t s, const Args& a, const Plan& p, float* seg_partial
     return H == 64 ? launch_fwd_valu<4, 64>(s, a, p, seg_partial) : launch_fwd_valu<4, 32>(s, a, p, seg_partial);
 }
 
-int dispatch_bwd(hipStream_t s, const Args& a, const Plan& p, const float* gproj, float* wt, float* dfeat,
-                 float4* sgc, unsigned int* dmax, float* partials, float* gpart, float* ggrid, float* gparams)
+int dispatch_bwd(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b, int stage)
 {
     const int C = a.m.in, H = a.m.hidden;
     if (a.mfma)
     {
         if (a.precision == NEAT_PREC_BF16)
-            return H == 64 ? launch_bw_mfma<PrecBF16, 64>(s, a, p, gproj, dfeat, sgc, dmax, partials, gpart, ggrid, gparams)
-                           : launch_bw_mfma<PrecBF16, 32>(s, a, p, gproj, dfeat, sgc, dmax, partials, gpart, ggrid, gparams);
-        return H == 64 ? launch_bw_mfma<PrecF32, 64>(s, a, p, gproj, dfeat, sgc, dmax, partials, gpart, ggrid, gparams)
-                       : launch_bw_mfma<PrecF32, 32>(s, a, p, gproj, dfeat, sgc, dmax, partials, gpart, ggrid, gparams);
+            return H == 64 ? launch_bw_mfma<PrecBF16, 64>(s, a, p, b, stage) : launch_bw_mfma<PrecBF16, 32>(s, a, p, b, stage);
+        return H == 64 ? launch_bw_mfma<PrecF32, 64>(s, a, p, b, stage) : launch_bw_mfma<PrecF32, 32>(s, a, p, b, stage);
     }
     if (C == 8)
-        return H == 64 ? launch_bw_valu<8, 64>(s, a, p, gproj, wt, dfeat, partials, gpart, ggrid, gparams)
-                       : launch_bw_valu<8, 32>(s, a, p, gproj, wt, dfeat, partials, gpart, ggrid, gparams);
-    return H == 64 ? launch_bw_valu<4, 64>(s, a, p, gproj, wt, dfeat, partials, gpart, ggrid, gparams)
-                   : launch_bw_valu<4, 32>(s, a, p, gproj, wt, dfeat, partials, gpart, ggrid, gparams);
+        return H == 64 ? launch_bw_valu<8, 64>(s, a, p, b, stage) : launch_bw_valu<8, 32>(s, a, p, b, stage);
+    return H == 64 ? launch_bw_valu<4, 64>(s, a, p, b, stage) : launch_bw_valu<4, 32>(s, a, p, b, stage);
 }
 
 // first node-major sample of every canonical segment
@@ -1596,33 +1719,59 @@ extern "C"
         return NEAT_OK;
     }
 
-    int neat_projector_bwd(neat_stream_t stream, const neat_projector_args* x, const float* grad_proj,
-                           float* grad_grid, float* grad_params, void* ws, size_t bytes)
+    int neat_projector_bwd_stage(neat_stream_t stream, const neat_projector_args* x, int32_t stage,
+                                 const float* grad_proj, float* grad_grid, float* grad_params, void* ws, size_t bytes)
     {
         Args a;
         int st = make_args(a, x);
         if (st) return st;
         if ((st = check_ws(a, ws, bytes))) return st;
-        NEAT_REQUIRE(grad_grid && grad_params && (grad_proj || x->num_rays == 0), "projector: null gradient buffer");
-        hipStream_t s    = as_stream(stream);
-        const auto& z    = a.z;
-        Plan p           = plan_view(ws, z);
-        float* wt        = reinterpret_cast<float*>(region(ws, z, 12));
-        float* dfeat     = reinterpret_cast<float*>(region(ws, z, 13));
-        const Layout L   = proj_layout(z);
-        float* partials  = reinterpret_cast<float*>(static_cast<char*>(ws) + L.total);
-        float* gpartials = reinterpret_cast<float*>(static_cast<char*>(ws) + L.total + partials_bytes(z));
+        NEAT_REQUIRE(stage >= 1 && stage <= 3, "neat_projector_bwd_stage: stage %d not in 1..3", stage);
+        NEAT_REQUIRE((stage != 1 || grad_proj || x->num_rays == 0) && (stage != 2 || grad_params) &&
+                         (stage != 3 || grad_grid),
+                     "projector: null gradient buffer");
+        hipStream_t s   = as_stream(stream);
+        const auto& z   = a.z;
+        Plan p          = plan_view(ws, z);
+        const Layout L  = proj_layout(z);
         if (z.K == 0)
         {
             const int64_t ng = z.N * z.G * z.G * z.G * z.C;
-            k_fill_f32<<<grid_for(ng, 256), 256, 0, s>>>(grad_grid, ng, 0.f);
-            NEAT_LAUNCHED();
-            k_fill_f32<<<grid_for(z.P, 256), 256, 0, s>>>(grad_params, z.P, 0.f);
-            NEAT_LAUNCHED();
+            if (stage == 3)
+            {
+                k_fill_f32<<<grid_for(ng, 256), 256, 0, s>>>(grad_grid, ng, 0.f);
+                NEAT_LAUNCHED();
+            }
+            if (stage == 2)
+            {
+                k_fill_f32<<<grid_for(z.P, 256), 256, 0, s>>>(grad_params, z.P, 0.f);
+                NEAT_LAUNCHED();
+            }
             return NEAT_OK;
         }
-        float4* sgc      = reinterpret_cast<float4*>(region(ws, z, 14));
-        unsigned int* dm = reinterpret_cast<unsigned int*>(region(ws, z, 15));
-        return dispatch_bwd(s, a, p, grad_proj, wt, dfeat, sgc, dm, partials, gpartials, grad_grid, grad_params);
+        BwdBufs b;
+        b.grad_proj     = grad_proj;
+        b.wt            = reinterpret_cast<float*>(region(ws, z, 12));
+        b.dfeat         = reinterpret_cast<float*>(region(ws, z, 13));
+        b.sgc           = reinterpret_cast<float4*>(region(ws, z, 14));
+        b.dmax          = reinterpret_cast<unsigned int*>(region(ws, z, 15));
+        b.partials      = reinterpret_cast<float*>(static_cast<char*>(ws) + L.total);
+        b.grid_partials = reinterpret_cast<float*>(static_cast<char*>(ws) + L.total + partials_bytes(z));
+        b.grad_grid     = grad_grid;
+        b.grad_params   = grad_params;
+        return dispatch_bwd(s, a, p, b, stage);
+    }
+
+    int neat_projector_bwd(neat_stream_t stream, const neat_projector_args* x, const float* grad_proj,
+                           float* grad_grid, float* grad_params, void* ws, size_t bytes)
+    {
+        NEAT_REQUIRE(grad_grid && grad_params && (grad_proj || (x && x->num_rays == 0)),
+                     "projector: null gradient buffer");
+        for (int stage = 1; stage <= 3; ++stage)
+        {
+            int st = neat_projector_bwd_stage(stream, x, stage, grad_proj, grad_grid, grad_params, ws, bytes);
+            if (st) return st;
+        }
+        return NEAT_OK;
     }
 }

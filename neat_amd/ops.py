@@ -228,6 +228,12 @@ class Projector:
              ptr(self.ws), self.ws.numel())
         return proj
 
+    def backward_stage(self, stage, grad_proj, grad_grid, grad_params):
+        """One stage of the backward (1 decoder, 2 parameter gradient, 3 grid gradient); 1..3 in order = backward."""
+        call("neat_projector_bwd_stage", stream_ptr(self.stream), ctypes.byref(self.args), int(stage),
+             ptr(grad_proj) if grad_proj is not None else None, ptr(grad_grid), ptr(grad_params), ptr(self.ws),
+             self.ws.numel())
+
     def sample_values(self):
         """(values [S] in the plan's node-major order, seg_value_offset [K]): the decoded field at every sample."""
         values = torch.empty(max(self.S, 1), dtype=_F32, device=self.grid.device)
