@@ -51,6 +51,7 @@ struct ProjGeom
     int32_t G;
     float beta;
     const float* seg_cdf;  // optional [K][8] proposal CDF (row used when cdf[7] > 0)
+    int64_t S;             // samples of the segment list
 };
 
 struct Plan
@@ -627,37 +628,44 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
         // The backward kept every sample's grid coordinate. Channel half blockIdx.y of the node grid is accumulated
         // in 64-bit fixed point (value x 2^F, F from the batch's largest |dL/dfeature|): integer adds are exact and
         // order-independent, so the result is bit-reproducible, and LDS integer atomics run several times faster
-        // than float ones. Lanes are (corner, channel) pairs of 2 samples, so one instruction's 64 adds hit 64
-        // distinct addresses; each wave streams a contiguous run of the chunk's samples, one sample ahead.
+        // than float ones. Lanes are (sample stream, corner) pairs: the 8 lanes of a stream share its sample's loads,
+        // each computes one corner's weight and vertex once and adds all 4 channels of the half. The chunk's samples
+        // are cut into 8 x 16 contiguous streams.
         unsigned long long* sq = reinterpret_cast<unsigned long long*>(sg);
-        const int sub = lane >> 5, kk = (lane >> 2) & 7, c4 = lane & 3;
-        const int64_t U2 = (int64_t)(blockDim.x >> 6) * 2;
+        const int sub = lane >> 3, kk = lane & 7;
+        const int64_t U2 = (int64_t)(blockDim.x >> 6) * 8;
         const int64_t L2 = (ns + U2 - 1) / U2;
-        int64_t s2       = ((int64_t)wave * 2 + sub) * L2;
+        int64_t s2       = ((int64_t)sub * (blockDim.x >> 6) + wave) * L2;  // far-apart streams share a wave
         const int64_t e2 = min(ns, s2 + L2);
         const double scale = fixed_scale(*dmax);
-        const int ch0      = 4 * (int)blockIdx.y;
+        const float4* dh   = reinterpret_cast<const float4*>(dfeat) + (int64_t)blockIdx.y * g.S + base;
         if (s2 < e2)
         {
-            // Consecutive samples of a stream usually fall in the same cell: a lane keeps its (corner, channel) sum
-            // in a register while the address stays the same and adds it to LDS when the address changes (exact:
-            // the sum is an integer either way). Loads go in blocks of kSB samples, the next block in flight while
-            // the current one is accumulated.
-            constexpr int kSB = 8;
-            int addr      = -1;
-            long long acc = 0;
-            auto load = [&](int64_t s0, float4 (&gb)[kSB], float (&db)[kSB]) {
+            // Consecutive samples of a stream usually fall in the same cell: a lane keeps its corner's 4 sums in
+            // registers while the vertex stays the same and adds them to LDS when it changes (exact: integers either
+            // way). Loads go in blocks of kSB samples, the next block in flight while the current one is summed.
+            constexpr int kSB = 4;
+            int vtx = -1;
+            long long acc[4] = {0, 0, 0, 0};
+            auto flush = [&]() {
+                if (vtx >= 0)
+                {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) atomicAdd(sq + vtx * 4 + q, (unsigned long long)acc[q]);
+                }
+            };
+            auto load = [&](int64_t s0, float4 (&gb)[kSB], float4 (&db)[kSB]) {
 #pragma unroll
                 for (int b = 0; b < kSB; ++b)
                 {
                     if (s0 + b < e2)
                     {
                         gb[b] = sgc[base + s0 + b];
-                        db[b] = dfeat[(base + s0 + b) * C + ch0 + c4];
+                        db[b] = dh[s0 + b];
                     }
                 }
             };
-            auto accumulate = [&](int64_t s0, const float4 (&gb)[kSB], const float (&db)[kSB]) {
+            auto accumulate = [&](int64_t s0, const float4 (&gb)[kSB], const float4 (&db)[kSB]) {
 #pragma unroll
                 for (int b = 0; b < kSB; ++b)
                 {
@@ -672,19 +680,21 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
                     const float wk = fx * fy * fz;
                     const int cx = clamp_g((kk & 1) ? x1 : x0, g.G), cy = clamp_g((kk & 2) ? y1 : y0, g.G),
                               cz = clamp_g((kk & 4) ? z1 : z0, g.G);
-                    const long long q = __double2ll_rn((double)(wk * db[b]) * scale);
-                    const int a       = ((cz * g.G + cy) * g.G + cx) * 4 + c4;
-                    if (a != addr)
+                    const int v = (cz * g.G + cy) * g.G + cx;
+                    if (v != vtx)
                     {
-                        if (addr >= 0) atomicAdd(sq + addr, (unsigned long long)acc);
-                        addr = a;
-                        acc  = 0;
+                        flush();
+                        vtx    = v;
+                        acc[0] = acc[1] = acc[2] = acc[3] = 0;
                     }
-                    acc += q;
+                    const float4 dv = db[b];
+                    acc[0] += __double2ll_rn((double)(wk * dv.x) * scale);
+                    acc[1] += __double2ll_rn((double)(wk * dv.y) * scale);
+                    acc[2] += __double2ll_rn((double)(wk * dv.z) * scale);
+                    acc[3] += __double2ll_rn((double)(wk * dv.w) * scale);
                 }
             };
-            float4 ga[kSB], gb[kSB];
-            float da[kSB], db[kSB];
+            float4 ga[kSB], gb[kSB], da[kSB], db[kSB];
             load(s2, ga, da);
             for (int64_t s0 = s2; s0 < e2; s0 += 2 * kSB)
             {
@@ -693,8 +703,9 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
                 load(s0 + 2 * kSB, ga, da);
                 accumulate(s0 + kSB, gb, db);
             }
-            if (addr >= 0) atomicAdd(sq + addr, (unsigned long long)acc);
+            flush();
         }
+        const int ch0 = 4 * (int)blockIdx.y;
         __syncthreads();
         const bool split = p.chunk_off2[slot + 1] - p.chunk_off2[slot] > 1;
         float* dst       = split ? partial_grids + c * gsize : grad_grid + (int64_t)slot * gsize;
@@ -1192,7 +1203,7 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
                     }
                     if (valid)
                     {
-                        *reinterpret_cast<float4*>(dfeat + (base + w0 + i) * C + 4 * hg) =
+                        *reinterpret_cast<float4*>(dfeat + ((int64_t)hg * g.S + base + w0 + i) * 4) =
                             make_float4(dx[0], dx[1], dx[2], dx[3]);
                         amax = fmaxf(amax, fmaxf(fmaxf(fabsf(dx[0]), fabsf(dx[1])), fmaxf(fabsf(dx[2]), fabsf(dx[3]))));
                     }
@@ -1318,7 +1329,7 @@ Layout proj_layout(const ProjSizes& z)
     L.add(std::max<size_t>(z.sort_temp, scan_workspace_bytes(z.K + 1)));  // 10 sort / scan temp
     L.add(4 * z.K);                       // 11 seg_partial (forward)
     L.add(4 * (size_t)z.P);               // 12 transposed decoder params
-    L.add(4 * (size_t)z.S * z.C);         // 13 dfeat
+    L.add(4 * (size_t)z.S * z.C);         // 13 dfeat ([S][C]; matrix-core backward: [C/4][S][4])
     L.add(16 * (size_t)z.S);              // 14 grid coordinates (ix, iy, iz, -) per sample, matrix-core backward
     L.add(256);                           // 15 max |dL/dfeature| (float bits), matrix-core backward
     return L;
@@ -1426,7 +1437,7 @@ int make_args(Args& a, const neat_projector_args* x)
     NEAT_REQUIRE((reinterpret_cast<uintptr_t>(x->seg_cdf) % 16) == 0, "projector: seg_cdf must be 16-byte aligned");
     a.g = ProjGeom{x->node_pos_min, x->node_pos_max, x->node_active_prefix_sum, x->num_nodes, x->ray_origin,
                    x->ray_direction, R, x->ray_seg_offset, K, x->seg_node, x->seg_ray, x->seg_count, x->seg_tmin,
-                   x->seg_tmax, x->jitter, x->seed, x->grid, N, G, x->softplus_beta, x->seg_cdf};
+                   x->seg_tmax, x->jitter, x->seed, x->grid, N, G, x->softplus_beta, x->seg_cdf, S};
     a.m         = MLP{x->in_features, x->hidden, x->hidden_layers, x->out_features};
     a.act       = x->act;
     a.params    = x->params;
