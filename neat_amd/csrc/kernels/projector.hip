@@ -17,6 +17,8 @@
 //              atomics anywhere.
 #include <hipcub/hipcub.hpp>
 
+#include <type_traits>
+
 #include "mfma_mlp.hpp"
 #include "mlp_tile.hpp"
 
@@ -1058,6 +1060,38 @@ __global__ void __launch_bounds__(256, 2) k_mfma_fwd(ProjGeom g, Plan p, const f
     }
 }
 
+// The backward's [row][sample] staging buffers: BF16 (rows of 40) for the BF16 form, where the values are rounded
+// once when staged and read as ready operands, FP32 (rows of kRS) otherwise.
+template <typename P>
+struct Stage
+{
+    static constexpr int kStride = P::kBF16 ? 40 : kRS;  // elements per row; 16-byte rows either way
+    typedef typename std::conditional<P::kBF16, unsigned short, float>::type E;
+    __device__ static __forceinline__ void put(E* b, int row, int col, float v)
+    {
+        if constexpr (P::kBF16)
+            b[row * kStride + col] = (unsigned short)bf16_bits(v);
+        else
+            b[row * kStride + col] = v;
+    }
+    // A / B fragment over the tile's samples (see stage_frag); masked lanes read row 0 and discard it
+    __device__ static __forceinline__ typename P::Frag frag(const E* b, int row, int st, int hg, bool zero)
+    {
+        if constexpr (P::kBF16)
+        {
+            const uint4 u = *reinterpret_cast<const uint4*>(b + (zero ? 0 : row) * kStride + 16 * st + 8 * hg);
+            typename P::Frag f;
+            __builtin_memcpy(&f, &u, sizeof(f));
+            if (zero) f = typename P::Frag{};
+            return f;
+        }
+        else
+        {
+            return zero ? 0.f : b[row * kStride + 2 * st + hg];
+        }
+    }
+};
+
 // Rows of the X staging tile: the 8 features, then the columns that turn the X product into the bias and output
 // gradients (E = dZ1 [x; 1]^T + dZ2 [e10]^T + h2 [dz e9]^T, with the B operand masked per product).
 constexpr int kXOnes1 = 8, kXDz = 9, kXOnes2 = 10, kXRows = 11;
@@ -1075,12 +1109,14 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
     constexpr int KSTEPS = 32 / (2 * P::KS);  // sample steps of a weight-gradient product (FP32 16, BF16 2)
     __shared__ WeightLDS<P, H> w;
     __shared__ ChunkCtx cx;
-    __shared__ __align__(16) float stage[4][(2 * H + kXRows) * kRS];
+    typedef Stage<P> ST;
+    typedef typename ST::E SE;
+    __shared__ __align__(16) SE stage[4][(2 * H + kXRows) * ST::kStride];
     load_weights<P, H, C>(params, w);
     const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
-    float* sA = stage[wave];
-    float* sB = sA + H * kRS;
-    float* sX = sB + H * kRS;
+    SE* sA = stage[wave];
+    SE* sB = sA + H * ST::kStride;
+    SE* sX = sB + H * ST::kStride;
     f32x16 dW2[MT][MT], E[MT];
 #pragma unroll
     for (int a = 0; a < MT; ++a)
@@ -1094,8 +1130,8 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
     // the constant rows of the X tile
     if (hg == 0)
     {
-        sX[kXOnes1 * kRS + n] = 1.f;
-        sX[kXOnes2 * kRS + n] = 1.f;
+        ST::put(sX, kXOnes1, n, 1.f);
+        ST::put(sX, kXOnes2, n, 1.f);
     }
     const int64_t nchunks = p.chunk_off1[g.N];
     for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x)
@@ -1125,8 +1161,8 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
                 if (hg == 0) b3 += dz;
                 // X tile and the first staged pair: dZ2 (A) and act(z1) (B)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) sX[(4 * hg + j) * kRS + n] = x4[j];
-                if (hg == 0) sX[kXDz * kRS + n] = dz;
+                for (int j = 0; j < 4; ++j) ST::put(sX, 4 * hg + j, n, x4[j]);
+                if (hg == 0) ST::put(sX, kXDz, n, dz);
                 f32x16 dZ2[MT];
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
@@ -1136,8 +1172,8 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
                     {
                         const int r    = 32 * mt + crow(i2, hg);
                         dZ2[mt][i2]    = w.vec[2][r] * dz * act_d(ACT, f.z2[mt][i2], g.beta);
-                        sA[r * kRS + n] = dZ2[mt][i2];
-                        sB[r * kRS + n] = act_f(ACT, f.z1[mt][i2], g.beta);
+                        ST::put(sA, r, n, dZ2[mt][i2]);
+                        ST::put(sB, r, n, act_f(ACT, f.z1[mt][i2], g.beta));
                     }
                 }
                 wave_lds_fence();
@@ -1145,15 +1181,17 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
 #pragma unroll
                 for (int st = 0; st < KSTEPS; ++st)
                 {
-                    const auto bx = stage_frag<P>(sX, n, st, hg, n != kXOnes2);
+                    const auto bx = ST::frag(sX, n, st, hg, n != kXOnes2);
+                    typename P::Frag bf[MT];
+#pragma unroll
+                    for (int m1 = 0; m1 < MT; ++m1) bf[m1] = ST::frag(sB, 32 * m1 + n, st, hg, false);
 #pragma unroll
                     for (int m2 = 0; m2 < MT; ++m2)
                     {
-                        const auto a = stage_frag<P>(sA, 32 * m2 + n, st, hg, false);
+                        const auto a = ST::frag(sA, 32 * m2 + n, st, hg, false);
                         E[m2]        = P::mma(a, bx, E[m2]);
 #pragma unroll
-                        for (int m1 = 0; m1 < MT; ++m1)
-                            dW2[m2][m1] = P::mma(a, stage_frag<P>(sB, 32 * m1 + n, st, hg, false), dW2[m2][m1]);
+                        for (int m1 = 0; m1 < MT; ++m1) dW2[m2][m1] = P::mma(a, bf[m1], dW2[m2][m1]);
                     }
                 }
                 wave_lds_fence();
@@ -1163,15 +1201,15 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
                 {
 #pragma unroll
                     for (int i2 = 0; i2 < 16; ++i2)
-                        sB[(32 * mt + crow(i2, hg)) * kRS + n] = act_f(ACT, f.z2[mt][i2], g.beta);
+                        ST::put(sB, 32 * mt + crow(i2, hg), n, act_f(ACT, f.z2[mt][i2], g.beta));
                 }
                 wave_lds_fence();
 #pragma unroll
                 for (int st = 0; st < KSTEPS; ++st)
                 {
-                    const auto bx = stage_frag<P>(sX, n, st, hg, n != kXDz);
+                    const auto bx = ST::frag(sX, n, st, hg, n != kXDz);
 #pragma unroll
-                    for (int mt = 0; mt < MT; ++mt) E[mt] = P::mma(stage_frag<P>(sB, 32 * mt + n, st, hg, false), bx, E[mt]);
+                    for (int mt = 0; mt < MT; ++mt) E[mt] = P::mma(ST::frag(sB, 32 * mt + n, st, hg, false), bx, E[mt]);
                 }
                 // dZ1 = (W1^T dZ2) * act'(z1), in registers
                 f32x16 dZ1[MT];
@@ -1213,15 +1251,15 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
                 for (int mt = 0; mt < MT; ++mt)
                 {
 #pragma unroll
-                    for (int i2 = 0; i2 < 16; ++i2) sA[(32 * mt + crow(i2, hg)) * kRS + n] = dZ1[mt][i2];
+                    for (int i2 = 0; i2 < 16; ++i2) ST::put(sA, 32 * mt + crow(i2, hg), n, dZ1[mt][i2]);
                 }
                 wave_lds_fence();
 #pragma unroll
                 for (int st = 0; st < KSTEPS; ++st)
                 {
-                    const auto bx = stage_frag<P>(sX, n, st, hg, n > kXOnes1);
+                    const auto bx = ST::frag(sX, n, st, hg, n > kXOnes1);
 #pragma unroll
-                    for (int mt = 0; mt < MT; ++mt) E[mt] = P::mma(stage_frag<P>(sA, 32 * mt + n, st, hg, false), bx, E[mt]);
+                    for (int mt = 0; mt < MT; ++mt) E[mt] = P::mma(ST::frag(sA, 32 * mt + n, st, hg, false), bx, E[mt]);
                 }
                 wave_lds_fence();
             };
@@ -1229,8 +1267,16 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
             {
                 PairIn cur;
                 fetch_pair<P::kBF16, true>(g, cx, p0, w0, wn, base, mn, mx, ngrid, sgc, cur);
-                tile(p0 + n, cur.xa, cur.va);
-                if (p0 + 32 < wn) tile(p0 + 32 + n, cur.xb, cur.vb);
+                // one copy of the tile code for both halves of the pair (smaller code, fewer live registers)
+#pragma unroll 1
+                for (int h = 0; h < 2; ++h)
+                {
+                    if (h == 1 && p0 + 32 >= wn) break;
+                    float xs[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) xs[j] = h ? cur.xb[j] : cur.xa[j];
+                    tile(p0 + 32 * h + n, xs, h ? cur.vb : cur.va);
+                }
             }
             __syncthreads();
         }
