@@ -37,6 +37,18 @@ def parse():
     ap.add_argument("--cpu-rays", type=int, default=12288, help="rays in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--proposal", action="store_true",
+                    help="BF16 + learned proposal form: the field is the phantom fitted on the leaf grids, segments "
+                         "with more than 8 uniform samples get 8 proposal-guided ones, and the proposal network takes "
+                         "one online training step every --proposal-train-every steps on 1/32 of the batch's rays; reports the projection error of the "
+                         "form against the FP64 oracle with uniform samples")
+    ap.add_argument("--proposal-tau", type=float, default=0.3,
+                    help="fallback threshold: segments in slots whose observed field variation exceeds it keep their "
+                         "uniform samples")
+    ap.add_argument("--proposal-train-every", type=int, default=4,
+                    help="the proposal network's online training step runs every this many steps")
+    ap.add_argument("--proposal-pretrain", type=int, default=200,
+                    help="untimed proposal training steps (4096 rays each) before the timed region")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference kernel for scaling HBM "
                          "byte counters) and exit")
@@ -69,6 +81,10 @@ def main():
     cfg = W.C2
     dev = "cuda"
     scene = W.make_scene(cfg, seed=a.seed)
+    if a.proposal:
+        if a.path != "fused":
+            raise SystemExit("--proposal needs the fused path")
+        scene["grid"], scene["params"] = W.phantom_field(scene, cfg)
     ds = pipeline.to_device(scene, cfg, dev)
     batches = []
     for b in range(a.batches):
@@ -81,6 +97,18 @@ def main():
     grid_bf16 = ops.grid_to_bf16(ds.grid) if (a.path == "fused" and a.precision == "bf16") else None
     stage_names = (["sample", "plan", "proj_fwd", "loss", "bwd_decoder", "bwd_params", "bwd_grid"]
                    if a.path == "fused" else ["sample", "gs_fwd", "dec_fwd", "integrate", "dec_bwd", "gs_bwd"])
+    prop = None
+    if a.proposal:
+        from neat_amd import proposal as PP
+        stage_names = ["sample", "proposal", "proposal_train"] + stage_names[1:]
+        prop = PP.ProposalSampler(dev, num_samples=8, lr=3e-3, seed=a.seed + 7, num_slots=ds.grid.shape[0],
+                                  tau=a.proposal_tau)
+        for k in range(a.proposal_pretrain):  # untimed set-up: the network starts trained
+            o, d = W.batch_rays(cfg, 4096, seed=a.seed + 5000 + k)
+            to, td = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+            prop.train_step(ds, to, td, pipeline.sample(ds, to, td, jitter=False, emit=False), precision="bf16",
+                            grid_bf16=grid_bf16)
+        torch.cuda.synchronize()
     events = {k: [] for k in stage_names}
     gg_buf = torch.empty_like(ds.grid)
     gp_buf = torch.empty_like(ds.params)
@@ -92,6 +120,16 @@ def main():
         rec("sample", 0)
         segs = pipeline.sample(ds, o, d, jitter=True, seed=i, emit=False)
         rec("sample", 1)
+        if prop is not None:
+            rec("proposal", 0)
+            segs = prop.apply(o, d, segs, ds.prefix)
+            rec("proposal", 1)
+            rec("proposal_train", 0)
+            if i % a.proposal_train_every == 0:
+                so, sd = o[::32].contiguous(), d[::32].contiguous()
+                prop.train_step(ds, so, sd, pipeline.sample(ds, so, sd, jitter=False, emit=False), precision="bf16",
+                                grid_bf16=grid_bf16)
+            rec("proposal_train", 1)
         rec("plan", 0)
         pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, o, d, segs, ds.grid, ds.params, ds.mlp, True, i,
                            precision=a.precision, grid_bf16=grid_bf16)
@@ -179,6 +217,10 @@ def main():
         "bwd_grid": ("hbm", 48 * mean_samples + grid_bytes, "GB/s", HBM_PEAK_GBS, f"k_proj_bw2<{C}>"),
         "bwd_params": ("hbm", 4.0 * 1024 * ds.params.numel(), "GB/s", HBM_PEAK_GBS, "k_dec_reduce_partials"),
         "plan": ("hbm", 16 * mean_samples / 27, "GB/s", HBM_PEAK_GBS, "plan"),
+        "proposal": ("mfma", 2 * (6 * 32 + 32 * 32 + 32 * 8) * mean_samples / 8, "TFLOP/s", BF16_PEAK_TFLOPS,
+                     "k_prop_fwd"),
+        "proposal_train": ("mfma", 3 * 2 * (6 * 32 + 32 * 32 + 32 * 8) * mean_samples / 128, "TFLOP/s",
+                           BF16_PEAK_TFLOPS, "k_prop_bwd"),
         "loss": ("hbm", 12 * cfg.batch, "GB/s", HBM_PEAK_GBS, "k_mse"),
         "dec_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS, "k_dec_fwd"),
         "dec_bwd": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS, "k_dec_bwd"),
@@ -191,14 +233,16 @@ def main():
     secs = stage_ms[dominant] / 1e3
     achieved = amount / secs / (1e12 if unit == "TFLOP/s" else 1e9)
     roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                "frac": round(achieved / peak, 5), "traffic": measured_traffic(kname), "kernel": kname,
+                "frac": round(achieved / peak, 5),
+                "traffic": None if prop is not None else measured_traffic(kname), "kernel": kname,
                 "stage": dominant, "kernel_ms": round(stage_ms[dominant], 4)}
 
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(scene, cfg, a.cpu_rays, a.seed)
+    err = projection_error(scene, cfg, ds, prop, a, grid_bf16)
 
-    prec_name = a.precision if a.path == "fused" else "fp32_unfused"
+    prec_name = (a.precision + ("+proposal" if prop is not None else "")) if a.path == "fused" else "fp32_unfused"
     out = {
         "metric": "fwd+bwd rays/s at 1 GPU, BF16+proposal vs plain FP32, at <=1e-3 rel err vs FP64",
         "value": round(rays_per_s, 1),
@@ -216,12 +260,39 @@ def main():
                                "decoder 8->64->64->1, 256x512^2 cone beam, batch 2^18 rays, max 64 samples/node",
                    "path": a.path, "precision": prec_name, "batch_rays": cfg.batch, "samples_per_batch": round(mean_samples),
                    "samples_per_s": round(mean_samples / (ms_per_step / 1e3), 1),
+                   "field": "phantom fitted on the leaf grids" if prop is not None else "random grid N(0, 0.1)",
+                   "projection_error_vs_fp64": err,
                    "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def projection_error(scene, cfg, ds, prop, a, grid_bf16, nrays=2048):
+    """Relative projection error of the benchmarked form against the FP64 oracle with the reference's uniform samples,
+    on a fixed sub-batch (no jitter), with the error bar of the metric."""
+    if a.path != "fused":
+        return None
+    import torch
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from neat_amd import ops, pipeline, workloads as W
+    from oracle import oracle as O
+    o, d = W.batch_rays(cfg, nrays, seed=a.seed + 777)
+    to, td = torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda()
+    segs = pipeline.sample(ds, to, td, jitter=False, emit=False)
+    hseg = {k: v.cpu().numpy() for k, v in segs.items() if torch.is_tensor(v)}
+    ref = O.projector_fwd(scene["tree"], scene["prefix"], o, d, hseg, scene["grid"], dict(scene["mlp"],
+                          params=scene["params"]), prec=1)
+    if prop is not None:
+        segs = prop.apply(to, td, segs, ds.prefix)
+    pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, to, td, segs, ds.grid, ds.params, ds.mlp, False, 0,
+                       precision=a.precision, grid_bf16=grid_bf16)
+    p = pj.forward().cpu().numpy().astype(np.float64)
+    e = float(np.linalg.norm(p - ref) / np.linalg.norm(ref))
+    return {"rel": e, "bar": 1e-3, "meets_bar": e <= 1e-3, "rays": nrays,
+            "judge": "FP64 oracle, uniform samples (max 64 per node), same segments"}
 
 
 def measured_traffic(kernel):

@@ -297,6 +297,20 @@ int neat_proposal_fwd(neat_stream_t stream, const float* ray_origin, const float
 int neat_proposal_targets(neat_stream_t stream, const float* values, const int64_t* seg_value_offset,
                           const int32_t* seg_count, int64_t num_segments, float* target);
 
+/* The fallback's table, one value per grid slot: every slot's value is multiplied by decay, then raised to the total
+ * variation sum_j |f_{j+1} - f_j| of the field's values along any of the given uniformly sampled segments that lies in
+ * the slot (values as for neat_proposal_targets). */
+int neat_proposal_roughness_update(neat_stream_t stream, const float* values, const int64_t* seg_value_offset,
+                                   const int32_t* seg_count, const int32_t* seg_node, int64_t num_segments,
+                                   const int64_t* node_active_prefix_sum, int32_t num_nodes, int64_t num_slots,
+                                   float decay, float* slot_roughness);
+
+/* Uniform in-node sampling as the fallback: segments whose slot's roughness exceeds tau get a zero CDF row and their
+ * uniform count back (seg_count, the sampler's counts) in seg_count_out; the others keep what neat_proposal_fwd set. */
+int neat_proposal_gate(neat_stream_t stream, const int32_t* seg_node, const int32_t* seg_count, int64_t num_segments,
+                       const int64_t* node_active_prefix_sum, int32_t num_nodes, int64_t num_slots,
+                       const float* slot_roughness, float tau, float* seg_cdf, int32_t* seg_count_out);
+
 /* grad_params = gradient of scale * sum_k CE(target_k, softmax(logits_k)) (scale = 1 / num_segments for the mean). */
 size_t neat_proposal_workspace_bytes(int64_t num_segments);
 int neat_proposal_bwd(neat_stream_t stream, const float* ray_origin, const float* ray_direction, const int32_t* seg_ray,

@@ -356,6 +356,46 @@ __global__ void __launch_bounds__(256) k_prop_targets(const float* values, const
     out[1]      = make_float4(tv[4] / s, tv[5] / s, tv[6] / s, tv[7] / s);
 }
 
+// Per-slot roughness: decay, then the largest total variation of the field along any observed segment of the slot.
+__global__ void __launch_bounds__(256) k_rough_decay(float* rough, int64_t n, float decay)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rough[i] *= decay;
+}
+
+__global__ void __launch_bounds__(256) k_rough_update(const float* values, const int64_t* off, const int32_t* count,
+                                                      const int32_t* seg_node, const int64_t* prefix, int32_t num_nodes,
+                                                      int64_t num_slots, int64_t K, float* rough)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int32_t node = seg_node[k];
+    if (node < 0 || node >= num_nodes) return;
+    const int64_t slot = prefix[node];
+    if (slot < 0 || slot >= num_slots) return;
+    const float* f = values + off[k];
+    float tv       = 0.f;
+    for (int32_t j = 0; j + 1 < count[k]; ++j) tv += fabsf(f[j + 1] - f[j]);
+    atomicMax(reinterpret_cast<unsigned int*>(rough) + slot, __float_as_uint(tv));  // non-negative: bits order
+}
+
+// Fallback to the uniform samples for segments whose slot is rougher than tau.
+__global__ void __launch_bounds__(256) k_prop_gate(const int32_t* seg_node, const int64_t* prefix, int32_t num_nodes,
+                                                   int64_t num_slots, const float* rough, float tau,
+                                                   const int32_t* count_in, int64_t K, float* cdf, int32_t* count_out)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int32_t node = seg_node[k];
+    const int64_t slot = (node >= 0 && node < num_nodes) ? prefix[node] : -1;
+    const bool rough_slot = slot < 0 || slot >= num_slots || rough[slot] > tau;
+    if (!rough_slot) return;
+    float4* r = reinterpret_cast<float4*>(cdf + 8 * k);
+    r[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+    r[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    count_out[k] = count_in[k];
+}
+
 int prop_bwd_workgroups(int64_t K) { return (int)std::max<int64_t>(1, std::min<int64_t>((K + 127) / 128, 256)); }
 
 int check_segs(const PropSegs& s, const float* params)
@@ -420,6 +460,47 @@ extern "C"
         k_prop_bwd<<<nwg, 256, 0, hs>>>(s, params, target, scale, part);
         NEAT_LAUNCHED();
         k_dec_reduce_partials<<<grid_for(kPParams, 256), 256, 0, hs>>>(part, nwg * 4, kPParams, grad_params);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
+
+    int neat_proposal_roughness_update(neat_stream_t stream, const float* values, const int64_t* seg_value_offset,
+                                       const int32_t* seg_count, const int32_t* seg_node, int64_t num_segments,
+                                       const int64_t* node_active_prefix_sum, int32_t num_nodes, int64_t num_slots,
+                                       float decay, float* slot_roughness)
+    {
+        NEAT_REQUIRE(num_segments >= 0 && num_slots >= 0 && num_nodes >= 0 && decay >= 0.f && decay <= 1.f,
+                     "neat_proposal_roughness_update: bad arguments");
+        NEAT_REQUIRE(slot_roughness || num_slots == 0, "neat_proposal_roughness_update: null table");
+        hipStream_t s = as_stream(stream);
+        if (num_slots > 0)
+        {
+            k_rough_decay<<<grid_for(num_slots, 256), 256, 0, s>>>(slot_roughness, num_slots, decay);
+            NEAT_LAUNCHED();
+        }
+        if (num_segments == 0) return NEAT_OK;
+        NEAT_REQUIRE(values && seg_value_offset && seg_count && seg_node && node_active_prefix_sum,
+                     "neat_proposal_roughness_update: null pointer");
+        k_rough_update<<<grid_for(num_segments, 256), 256, 0, s>>>(values, seg_value_offset, seg_count, seg_node,
+                                                                   node_active_prefix_sum, num_nodes, num_slots,
+                                                                   num_segments, slot_roughness);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
+
+    int neat_proposal_gate(neat_stream_t stream, const int32_t* seg_node, const int32_t* seg_count,
+                           int64_t num_segments, const int64_t* node_active_prefix_sum, int32_t num_nodes,
+                           int64_t num_slots, const float* slot_roughness, float tau, float* seg_cdf,
+                           int32_t* seg_count_out)
+    {
+        NEAT_REQUIRE(num_segments >= 0, "neat_proposal_gate: bad segment count");
+        if (num_segments == 0) return NEAT_OK;
+        NEAT_REQUIRE(seg_node && seg_count && node_active_prefix_sum && slot_roughness && seg_cdf && seg_count_out,
+                     "neat_proposal_gate: null pointer");
+        NEAT_REQUIRE(reinterpret_cast<uintptr_t>(seg_cdf) % 16 == 0, "neat_proposal_gate: seg_cdf must be 16-byte aligned");
+        k_prop_gate<<<grid_for(num_segments, 256), 256, 0, as_stream(stream)>>>(
+            seg_node, node_active_prefix_sum, num_nodes, num_slots, slot_roughness, tau, seg_count, num_segments,
+            seg_cdf, seg_count_out);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }

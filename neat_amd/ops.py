@@ -297,3 +297,17 @@ def proposal_bwd(origin, direction, segs, params, target, scale=None, stream=Non
          ptr(segs["seg_tmin"]), ptr(segs["seg_tmax"]), K, ptr(params), ptr(target), float(scale), ptr(gp), ptr(ws),
          ws.numel())
     return gp
+
+
+def proposal_roughness_update(values, seg_value_offset, segs, prefix, num_slots, roughness, decay, stream=None):
+    """Decay the per-slot roughness table and raise it to the field's variation along the given uniform segments."""
+    K = segs["seg_count"].shape[0]
+    call("neat_proposal_roughness_update", stream_ptr(stream), ptr(values), ptr(seg_value_offset),
+         ptr(segs["seg_count"]), ptr(segs["seg_node"]), K, ptr(prefix), prefix.shape[0], int(num_slots),
+         float(decay), ptr(roughness))
+
+
+def proposal_gate(segs, seg_count_out, seg_cdf, prefix, num_slots, roughness, tau, stream=None):
+    K = segs["seg_count"].shape[0]
+    call("neat_proposal_gate", stream_ptr(stream), ptr(segs["seg_node"]), ptr(segs["seg_count"]), K, ptr(prefix),
+         prefix.shape[0], int(num_slots), ptr(roughness), float(tau), ptr(seg_cdf), ptr(seg_count_out))

@@ -27,17 +27,28 @@ def init_params(seed=0):
 
 
 class ProposalSampler:
-    def __init__(self, device="cuda", num_samples=8, lr=1e-3, seed=0):
+    """The network, its Adam state, and the fallback's per-slot roughness table (the largest variation of the field
+    along a segment seen in the slot, decayed every training step): segments in slots rougher than `tau` keep their
+    uniform samples (tau None: no fallback)."""
+
+    def __init__(self, device="cuda", num_samples=8, lr=1e-3, seed=0, num_slots=0, tau=None, decay=0.98):
         self.params = torch.from_numpy(init_params(seed)).to(device)
         self.exp_avg = torch.zeros_like(self.params)
         self.exp_avg_sq = torch.zeros_like(self.params)
         self.num_samples = num_samples
         self.lr = lr
         self.steps = 0
+        self.tau = tau
+        self.decay = decay
+        self.roughness = torch.zeros(max(num_slots, 1), dtype=torch.float32, device=device)
+        self.num_slots = num_slots
 
-    def apply(self, origin, direction, segs):
-        """The segment list with proposal counts and CDF rows (a new dict; the input is not modified)."""
+    def apply(self, origin, direction, segs, prefix=None):
+        """The segment list with proposal counts and CDF rows (a new dict; the input is not modified). With a
+        fallback threshold, `prefix` (node -> slot) is needed."""
         cdf, cnt, _ = ops.proposal_fwd(origin, direction, segs, self.params, self.num_samples)
+        if self.tau is not None:
+            ops.proposal_gate(segs, cnt, cdf, prefix, self.num_slots, self.roughness, self.tau)
         out = dict(segs)
         out["seg_count"] = cnt
         out["seg_cdf"] = cdf
@@ -49,6 +60,9 @@ class ProposalSampler:
         pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, origin, direction, segs_uniform, ds.grid, ds.params,
                            ds.mlp, False, 0, precision=precision, grid_bf16=grid_bf16)
         values, off = pj.sample_values()
+        if self.tau is not None:
+            ops.proposal_roughness_update(values, off, segs_uniform, ds.prefix, self.num_slots, self.roughness,
+                                          self.decay)
         return ops.proposal_targets(values, off, segs_uniform["seg_count"])
 
     def train_step(self, ds, origin, direction, segs_uniform, precision="bf16", grid_bf16=None):

@@ -150,3 +150,58 @@ def test_trained_proposal_beats_uniform_at_equal_budget():
           "(gain %.2fx)" % (s_full, s_prop, s_full / s_prop, e_prop, e_unif, e_unif / e_prop), flush=True)
     assert losses[-1] < losses[0]
     assert e_prop < e_unif
+
+
+def test_roughness_table_and_fallback_gate():
+    print("case: per-slot roughness table and uniform fallback gate", flush=True)
+    cfg, sc, ds, o, d, to, td, segs = c2_segments(500, 8)
+    pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, to, td, segs, ds.grid, ds.params, ds.mlp, False, 0)
+    values, off = pj.sample_values()
+    N = ds.grid.shape[0]
+    rough = torch.full((N,), 2.0, device=DEV)
+    ops.proposal_roughness_update(values, off, segs, ds.prefix, N, rough, 0.5)
+    h = host_segs(segs)
+    v, of = values.cpu().numpy(), off.cpu().numpy()
+    slot = sc["prefix"][h["seg_node"]]
+    ref = np.full(N, 1.0)  # 2.0 decayed by 0.5
+    for k in range(len(of)):
+        tv = np.abs(np.diff(v[of[k]:of[k] + h["seg_count"][k]].astype(np.float64))).sum()
+        ref[slot[k]] = max(ref[slot[k]], tv)
+    assert np.allclose(rough.cpu().numpy(), ref, rtol=1e-5, atol=1e-6)
+    tau = float(np.median(ref))
+    ps = proposal.ProposalSampler(DEV, num_samples=8, seed=3, num_slots=N, tau=tau)
+    ps.roughness.copy_(rough)
+    p = ps.apply(to, td, segs, ds.prefix)
+    cnt, cdf = p["seg_count"].cpu().numpy(), p["seg_cdf"].cpu().numpy()
+    rough_seg = ref[slot] > tau
+    assert np.array_equal(cnt[rough_seg], h["seg_count"][rough_seg]) and np.all(cdf[rough_seg] == 0)
+    smooth = ~rough_seg & (h["seg_count"] > 8)
+    assert smooth.any() and np.all(cnt[smooth] == 8) and np.all(cdf[smooth, 7] == 1)
+
+
+def test_proposal_with_fallback_meets_the_error_bar():
+    print("case: trained proposal + roughness fallback vs the FP64 uniform judge, phantom field", flush=True)
+    from oracle import oracle as O
+    cfg = W.C2
+    sc = W.make_scene(cfg)
+    sc["grid"], sc["params"] = W.phantom_field(sc, cfg)
+    ds = pipeline.to_device(sc, cfg)
+    ps = proposal.ProposalSampler(DEV, num_samples=8, lr=3e-3, seed=7, num_slots=ds.grid.shape[0], tau=0.3)
+    for step in range(200):
+        o, d = W.batch_rays(cfg, 4096, seed=2000 + step)
+        to, td = torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV)
+        ps.train_step(ds, to, td, pipeline.sample(ds, to, td, jitter=False, emit=False), precision="fp32")
+    o, d = W.batch_rays(cfg, 2048, seed=98)
+    to, td = torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV)
+    segs = pipeline.sample(ds, to, td, jitter=False, emit=False)
+    h = host_segs(segs)
+    judge = O.projector_fwd(sc["tree"], sc["prefix"], o, d, h, sc["grid"], dict(sc["mlp"], params=sc["params"]),
+                            prec=1)
+    psegs = ps.apply(to, td, segs, ds.prefix)
+    pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, to, td, psegs, ds.grid, ds.params, ds.mlp, False, 0)
+    e = rel(pj.forward().cpu().numpy(), judge)
+    s_full, s_prop = int(segs["ray_sample_offset"][-1].item()), psegs["num_samples"]
+    print("proposal + fallback (tau 0.3): error vs FP64 uniform judge %.3e, samples %d of %d (%.2fx fewer)" % (
+        e, s_prop, s_full, s_full / s_prop), flush=True)
+    assert e <= 1e-3
+    assert s_prop < s_full
