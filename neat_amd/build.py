@@ -66,7 +66,7 @@ def build_host(jobs=8):
     os.environ.setdefault("MAX_JOBS", str(jobs))
     build_dir = os.path.join(ROOT, "_build", "host")
     os.makedirs(build_dir, exist_ok=True)
-    mod = ce.load(
+    ce.load(
         name="_neat_host",
         sources=srcs,
         extra_include_paths=[os.path.join(ROOT, "include"), HDIR, "/opt/rocm/include"],
@@ -75,11 +75,13 @@ def build_host(jobs=8):
                        "-L" + os.path.join(os.path.dirname(ce.__file__), "..", "..", "lib"), "-lc10_hip"],
         build_directory=build_dir,
         verbose=False,
-        is_python_module=True,
+        # built and dlopened only: importing it here as a top-level module would register its pybind11 types once
+        # more when the package imports its in-tree copy (neat_amd._neat_host) in the same process
+        is_python_module=False,
     )
     # copy the built module next to the package so it travels with the tree and is loaded from there
     import shutil
-    so = mod.__file__
+    so = os.path.join(build_dir, "_neat_host.so")
     dst = os.path.join(HERE, "_neat_host.so")
     shutil.copyfile(so, dst + ".tmp")
     os.replace(dst + ".tmp", dst)
