@@ -206,14 +206,19 @@ def main():
     prec_tag = {"bf16": "PrecBF16", "fp32": "PrecF32"}.get(a.precision)
     act = 1 if cfg.act == "relu" else 2
     grid_bytes = ds.grid.numel() * 4
+    if not (mm and prec_tag):
+        bwd_kernel = f"k_proj_bw1<{C},{H}>"
+    elif prec_tag == "PrecBF16" and cfg.act == "relu":
+        bwd_kernel = f"k_mfma_bwd_relu<{H}>"
+    else:
+        bwd_kernel = f"k_mfma_bwd<{prec_tag},{H},{act}>"
     # algorithmic work per launch of each stage's dominant kernel (per-unit figure x the units of one launch):
     # decoder FLOPs per sample (backward: 2x the forward, no recomputation counted); the scatter moves 48 B per
     # sample (grid coordinate 16 B + dL/dfeature 32 B) and writes the gradient grid once
     work = {
         "proj_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", mm_peak,
                      f"k_mfma_fwd<{prec_tag},{H},{act}>" if mm and prec_tag else f"k_proj_fwd<{C},{H},{act}>"),
-        "bwd_decoder": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", mm_peak,
-                        f"k_mfma_bwd<{prec_tag},{H},{act}>" if mm and prec_tag else f"k_proj_bw1<{C},{H}>"),
+        "bwd_decoder": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", mm_peak, bwd_kernel),
         "bwd_grid": ("hbm", 48 * mean_samples + grid_bytes, "GB/s", HBM_PEAK_GBS, f"k_proj_bw2<{C}>"),
         "bwd_params": ("hbm", 4.0 * 1024 * ds.params.numel(), "GB/s", HBM_PEAK_GBS, "k_dec_reduce_partials"),
         "plan": ("hbm", 16 * mean_samples / 27, "GB/s", HBM_PEAK_GBS, "plan"),

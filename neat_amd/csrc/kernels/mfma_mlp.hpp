@@ -283,4 +283,92 @@ __device__ __forceinline__ typename P::Frag stage_frag(const float* buf, int row
     return f;
 }
 
+// ---- BF16 relu decoders: weights for products in both orientations --------------------------------------------
+// The A and B fragments of v_mfma_f32_32x32x16_bf16 share one register layout (lane l%32 = row of A / column of B,
+// lane l/32 and slot j = k), so mma(a, b) and mma(b, a) give a product and its transpose from the same registers.
+// Each weight fragment below therefore serves a product in either orientation. Biases ride in spare k slots as a
+// (hi, lo) pair of BF16 values against constant 1 slots, so they cost no elementwise adds.
+//   w1e[MT][64]          first layer: dst row 32mt + l%32; group 0 slots 0..3 = channels 0..3, 4 = bias hi,
+//                        5 = bias lo; group 1 slots 0..3 = channels 4..7
+//   w2[MT][MT][2][64]    hidden layer (as WeightLDS::w2)
+//   w2b[MT][64]          hidden bias: group 0 slots 0, 1 = bias hi, lo
+//   w2t[MT][MT][2][64]   transposed hidden layer with each source row scaled by its output-layer weight:
+//                        dst = first-layer unit 32mt + l%32, src = hidden unit 32ms + crow(8st + j, l/32)
+//   w1t[MT][2][64]       transposed first layer (as WeightLDS::w1t)
+//   w3[64], b3           output layer (FP32)
+template <int H>
+struct WeightLDSRelu
+{
+    static constexpr int MT = H / 32;
+    bf16x8 w1e[MT][64];
+    bf16x8 w2[MT][MT][2][64];
+    bf16x8 w2b[MT][64];
+    bf16x8 w2t[MT][MT][2][64];
+    bf16x8 w1t[MT][2][64];
+    float w3[64];
+    float b3;
+};
+
+// (hi, lo) BF16 split of an FP32 value into slots k, k + 1 of a fragment: hi + lo carries about 16 significant bits
+__device__ __forceinline__ void bf16_split(float v, bf16x8& f, int k)
+{
+    const __bf16 hi = (__bf16)v;
+    f[k]            = hi;
+    f[k + 1]        = (__bf16)(v - (float)hi);
+}
+
+// Packed parameter layout as load_weights (C = 8).
+template <int H>
+__device__ void load_weights_relu(const float* __restrict__ params, WeightLDSRelu<H>& w)
+{
+    constexpr int MT = H / 32, C = 8;
+    const float* W0 = params;
+    const float* b0 = W0 + H * C;
+    const float* W1 = b0 + H;
+    const float* b1 = W1 + H * H;
+    const float* W2 = b1 + H;
+    const float* b2 = W2 + H;
+    for (int t = threadIdx.x; t < MT * 64; t += blockDim.x)
+    {
+        const int mt = t / 64, l = t % 64, g = l >> 5, r = 32 * mt + (l & 31);
+        bf16x8 f = {}, fb = {};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) f[j] = (__bf16)W0[r * C + 4 * g + j];
+        if (g == 0)
+        {
+            bf16_split(b0[r], f, 4);
+            bf16_split(b1[r], fb, 0);
+        }
+        w.w1e[mt][l] = f;
+        w.w2b[mt][l] = fb;
+    }
+    for (int t = threadIdx.x; t < MT * MT * 2 * 64; t += blockDim.x)
+    {
+        const int l = t % 64, st = (t / 64) % 2, ms = (t / 128) % MT, mt = t / (128 * MT);
+        const int g = l >> 5, dst = 32 * mt + (l & 31);
+        bf16x8 a, b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+        {
+            const int src = 32 * ms + crow(st * 8 + j, g);
+            a[j]          = (__bf16)W1[dst * H + src];
+            b[j]          = (__bf16)(W1[src * H + dst] * W2[src]);
+        }
+        w.w2[mt][ms][st][l]  = a;
+        w.w2t[mt][ms][st][l] = b;
+    }
+    for (int t = threadIdx.x; t < MT * 2 * 64; t += blockDim.x)
+    {
+        const int l = t % 64, st = (t / 64) % 2, ms = t / 128;
+        const int g = l >> 5, c = l & 31;
+        bf16x8 f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) f[j] = (__bf16)(c < C ? W0[(32 * ms + crow(st * 8 + j, g)) * C + c] : 0.f);
+        w.w1t[ms][st][l] = f;
+    }
+    for (int t = threadIdx.x; t < 64; t += blockDim.x) w.w3[t] = t < H ? W2[t] : 0.f;
+    if (threadIdx.x == 0) w.b3 = b2[0];
+}
+
+
 }  // namespace neat

@@ -1315,6 +1315,298 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
     if (l == 0) atomicMax(dmax, __float_as_uint(amax));  // non-negative floats order like their bit patterns
 }
 
+// Backward for BF16 relu decoders (the configuration the projector benchmarks): as k_mfma_bwd, but every product is
+// formed on the matrix cores in the orientation its consumer needs (WeightLDSRelu), so no tile is transposed
+// through LDS. Per wave and tile of 32 samples, with "h" tiles holding hidden units in the registers and samples on
+// the lanes, and "s" tiles the transpose:
+//   forward  Z1h, Z1s = W0 [x;1]; H1 = relu(Z1) (BF16); Z2h, Z2s = W1 H1 + b1; z = w2 . relu(Z2h) + b2
+//   backward dz = dL/dz; D2 = dz * [Z2 > 0] (h and s; the output weights are folded into w2t and the final scale)
+//            dW1 += D2s H1s^T; E += D2s [e10] + relu(Z2s) [dz e9] + D1s [x; 1]  (bias / output / first-layer grads)
+//            D1h, D1s = (w2t D2h) * [H1 > 0]; dL/dfeature = W0^T D1h
+// The relu masks come from the sign of the BF16 activations, so only BF16 fragments stay live across the tile.
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+// waves per SIMD the relu backward is compiled for (its register budget: 512 / waves per lane)
+#ifndef NEAT_RELU_BWD_WAVES
+#define NEAT_RELU_BWD_WAVES 1
+#endif
+
+__device__ __forceinline__ bf16x8 relu_frag(const f32x16& z, int st)
+{
+    bf16x8 f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) f[j] = (__bf16)fmaxf(z[8 * st + j], 0.f);
+    return f;
+}
+
+// BF16 fragment of z (slots 8st..8st+7) where the relu activation h > 0, else 0
+__device__ __forceinline__ bf16x8 masked_frag(const f32x16& z, int st, const bf16x8& h)
+{
+    bf16x8 f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) f[j] = (__bf16)z[8 * st + j];
+    const s16x8 hb = __builtin_bit_cast(s16x8, h);
+    const s16x8 fb = __builtin_bit_cast(s16x8, f) & (hb > (short)0);
+    return __builtin_bit_cast(bf16x8, fb);
+}
+
+template <int H>
+__global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(ProjGeom g, Plan p, const float* __restrict__ params,
+                                                          const void* grid_lowp, const float* grad_proj, float* dfeat,
+                                                          float4* sgc, unsigned int* dmax, float* partials,
+                                                          int32_t P_total)
+{
+    constexpr int MT = H / 32, C = 8;
+    typedef PrecBF16 P;
+    __shared__ WeightLDSRelu<H> w;
+    __shared__ ChunkCtx cx;
+    __shared__ __align__(16) float s_dz[4][32];
+    load_weights_relu<H>(params, w);
+    __syncthreads();  // the final partial write reads w.w3 even in a workgroup that gets no chunk
+    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
+    // constant fragments: ONES (group 0 slots 0, 1) against the hidden bias; IDB routes the slots of the feature
+    // fragment to columns of the s-tile T (channels 0..7, 1 -> 8, 1 -> 10, dz -> 9)
+    bf16x8 ones = {}, idb = {};
+    if (hg == 0)
+    {
+        ones[0] = (__bf16)1.f;
+        ones[1] = (__bf16)1.f;
+    }
+    {
+        const int col[2][8] = {{0, 1, 2, 3, 8, 10, 9, -1}, {4, 5, 6, 7, -1, -1, -1, -1}};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) idb[j] = (__bf16)(col[hg][j] == n ? 1.f : 0.f);
+    }
+    f32x16 dW2[MT][MT], E[MT];
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+    {
+        E[a] = f32x16{};
+#pragma unroll
+        for (int b = 0; b < MT; ++b) dW2[a][b] = f32x16{};
+    }
+    float b3 = 0.f;
+    float amax = 0.f;
+    float* sdz = s_dz[wave];
+    const int64_t nchunks = p.chunk_off1[g.N];
+    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x)
+    {
+        const int32_t slot = p.chunk1_slot[c];
+        const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c], nseg = q1 - q0;
+        if (nseg == 0) continue;
+        const int64_t base = p.nm_soff[q0];
+        __syncthreads();
+        load_chunk_ctx(g, p, q0, q1, base, grad_proj, cx);
+        float mn[3], mx[3];
+        node_bounds(g, g.seg_node[p.perm[q0]], mn, mx);
+        const void* ngrid = node_grid_ptr(g, grid_lowp, true, slot);
+        __syncthreads();
+        const int32_t nsamp = cx.beg[nseg];
+        for (int32_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
+        {
+            const int32_t wn = min(kFwdChunk, nsamp - w0);
+            expand_window(cx, nseg, w0, wn);
+            __syncthreads();
+            auto tile = [&](int32_t i, const float (&x4)[4], float gy) {
+                const bool valid = i < wn;
+                // feature fragment: channels 4hg..4hg+3, group 0 adds the two bias slots (1, 1)
+                bf16x8 X = {};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) X[j] = (__bf16)x4[j];
+                if (hg == 0)
+                {
+                    X[4] = (__bf16)1.f;
+                    X[5] = (__bf16)1.f;
+                }
+                // first layer, both orientations
+                bf16x8 H1h[MT][2], H1s[MT][2];
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+                    const f32x16 zh = P::mma(w.w1e[mt][l], X, f32x16{});
+                    const f32x16 zs = P::mma(X, w.w1e[mt][l], f32x16{});
+#pragma unroll
+                    for (int st = 0; st < 2; ++st)
+                    {
+                        H1h[mt][st] = relu_frag(zh, st);
+                        H1s[mt][st] = relu_frag(zs, st);
+                    }
+                }
+                // hidden layer (h): output pre-activation and dz
+                bf16x8 D2h[MT][2];
+                float dz;
+                {
+                    f32x16 Z2h[MT];
+                    float part = 0.f;
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                    {
+                        f32x16 acc = P::mma(w.w2b[mt][l], ones, f32x16{});
+#pragma unroll
+                        for (int ms = 0; ms < MT; ++ms)
+                        {
+#pragma unroll
+                            for (int st = 0; st < 2; ++st) acc = P::mma(w.w2[mt][ms][st][l], H1h[ms][st], acc);
+                        }
+                        Z2h[mt] = acc;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                        {
+                            const float4 w3 = *reinterpret_cast<const float4*>(&w.w3[32 * mt + 8 * q + 4 * hg]);
+                            part = fmaf(w3.x, fmaxf(acc[4 * q + 0], 0.f), part);
+                            part = fmaf(w3.y, fmaxf(acc[4 * q + 1], 0.f), part);
+                            part = fmaf(w3.z, fmaxf(acc[4 * q + 2], 0.f), part);
+                            part = fmaf(w3.w, fmaxf(acc[4 * q + 3], 0.f), part);
+                        }
+                    }
+                    const float z = part + __shfl_xor(part, 32) + w.b3;
+                    dz = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, z, g.beta) : 0.f;
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                    {
+#pragma unroll
+                        for (int st = 0; st < 2; ++st)
+                        {
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) D2h[mt][st][j] = (__bf16)(Z2h[mt][8 * st + j] > 0.f ? dz : 0.f);
+                        }
+                    }
+                }
+                if (hg == 0) b3 += dz;
+                // dz per register of an s tile (sample crow(i, hg)), through LDS
+                wave_lds_fence();
+                if (hg == 0) sdz[n] = dz;
+                wave_lds_fence();
+                float dzs[16];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    const float4 v = *reinterpret_cast<const float4*>(&sdz[8 * q + 4 * hg]);
+                    dzs[4 * q + 0] = v.x;
+                    dzs[4 * q + 1] = v.y;
+                    dzs[4 * q + 2] = v.z;
+                    dzs[4 * q + 3] = v.w;
+                }
+                // the s tile T of [x; 1; dz; 1] (columns 0..7, 8, 9, 10) and its masked B fragments
+                bf16x8 B08[2], B9[2], B10[2];
+                {
+                    if (hg == 0) X[6] = (__bf16)dz;
+                    const f32x16 T = P::mma(X, idb, f32x16{});
+#pragma unroll
+                    for (int st = 0; st < 2; ++st)
+                    {
+                        bf16x8 t;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) t[j] = (__bf16)T[8 * st + j];
+                        B08[st] = n <= 8 ? t : bf16x8{};
+                        B9[st]  = n == 9 ? t : bf16x8{};
+                        B10[st] = n == 10 ? t : bf16x8{};
+                    }
+                }
+                // hidden layer (s): relu(Z2s) and D2s; output-weight and hidden-bias gradients; hidden-weight gradient
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+                    f32x16 acc = P::mma(ones, w.w2b[mt][l], f32x16{});
+#pragma unroll
+                    for (int ms = 0; ms < MT; ++ms)
+                    {
+#pragma unroll
+                        for (int st = 0; st < 2; ++st) acc = P::mma(H1h[ms][st], w.w2[mt][ms][st][l], acc);
+                    }
+#pragma unroll
+                    for (int st = 0; st < 2; ++st)
+                    {
+                        bf16x8 h2 = relu_frag(acc, st), d2;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) d2[j] = (__bf16)(acc[8 * st + j] > 0.f ? dzs[8 * st + j] : 0.f);
+                        E[mt] = P::mma(h2, B9[st], E[mt]);
+                        E[mt] = P::mma(d2, B10[st], E[mt]);
+#pragma unroll
+                        for (int m1 = 0; m1 < MT; ++m1) dW2[mt][m1] = P::mma(d2, H1s[m1][st], dW2[mt][m1]);
+                    }
+                }
+                // first layer (s): D1s and the first-layer weight / bias gradients
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+                    f32x16 acc = {};
+#pragma unroll
+                    for (int ms = 0; ms < MT; ++ms)
+                    {
+#pragma unroll
+                        for (int st = 0; st < 2; ++st) acc = P::mma(D2h[ms][st], w.w2t[mt][ms][st][l], acc);
+                    }
+#pragma unroll
+                    for (int st = 0; st < 2; ++st) E[mt] = P::mma(masked_frag(acc, st, H1s[mt][st]), B08[st], E[mt]);
+                }
+                // first layer (h): D1h and dL/dfeature = W0^T D1h (rows 0..7: registers 0..3 of each lane group)
+                f32x16 dx = {};
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+                    f32x16 acc = {};
+#pragma unroll
+                    for (int ms = 0; ms < MT; ++ms)
+                    {
+#pragma unroll
+                        for (int st = 0; st < 2; ++st) acc = P::mma(w.w2t[mt][ms][st][l], D2h[ms][st], acc);
+                    }
+#pragma unroll
+                    for (int st = 0; st < 2; ++st) dx = P::mma(w.w1t[mt][st][l], masked_frag(acc, st, H1h[mt][st]), dx);
+                }
+                if (valid)
+                {
+                    *reinterpret_cast<float4*>(dfeat + ((int64_t)hg * g.S + base + w0 + i) * 4) =
+                        make_float4(dx[0], dx[1], dx[2], dx[3]);
+                    amax = fmaxf(amax, fmaxf(fmaxf(fabsf(dx[0]), fabsf(dx[1])), fmaxf(fabsf(dx[2]), fabsf(dx[3]))));
+                }
+            };
+            for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
+            {
+                PairIn cur;
+                fetch_pair<true, true>(g, cx, p0, w0, wn, base, mn, mx, ngrid, sgc, cur);
+#pragma unroll 1
+                for (int h = 0; h < 2; ++h)
+                {
+                    if (h == 1 && p0 + 32 >= wn) break;
+                    float xs[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) xs[j] = h ? cur.xb[j] : cur.xa[j];
+                    tile(p0 + 32 * h + n, xs, h ? cur.vb : cur.va);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // ---- this wave's slice of the partial parameter gradient (packed layout); rows scaled by the output weights
+    // where the output weights were folded out of D2 ----
+    float* part   = partials + ((int64_t)blockIdx.x * 4 + wave) * P_total;
+    const int oW0 = 0, ob0 = H * C, oW1 = ob0 + H, ob1 = oW1 + H * H, oW2 = ob1 + H, ob2 = oW2 + H;
+#pragma unroll
+    for (int m2 = 0; m2 < MT; ++m2)
+    {
+#pragma unroll
+        for (int i2 = 0; i2 < 16; ++i2)
+        {
+            const int r    = 32 * m2 + crow(i2, hg);
+            const float s3 = w.w3[r];
+#pragma unroll
+            for (int m1 = 0; m1 < MT; ++m1) part[oW1 + r * H + 32 * m1 + n] = dW2[m2][m1][i2] * s3;
+            if (n < C) part[oW0 + r * C + n] = E[m2][i2];
+            if (n == 8) part[ob0 + r] = E[m2][i2];
+            if (n == 9) part[oW2 + r] = E[m2][i2];
+            if (n == 10) part[ob1 + r] = E[m2][i2] * s3;
+        }
+    }
+#pragma unroll
+    for (int off = 16; off >= 1; off >>= 1) b3 += __shfl_xor(b3, off);
+    if (l == 0) part[ob2] = b3;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off));
+    if (l == 0) atomicMax(dmax, __float_as_uint(amax));
+}
+
 // ---- workspace layout ---------------------------------------------------------------------------------------------------
 struct Layout
 {
@@ -1384,11 +1676,16 @@ Layout proj_layout(const ProjSizes& z)
 int bw1_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 2048)); }
 // matrix-core backward: persistent workgroups of 4 waves, one partial slice per wave
 int mfma_bwd_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 256)); }
+// k_mfma_bwd_relu: NEAT_RELU_BWD_WAVES workgroups per CU
+int mfma_bwd_relu_workgroups(const ProjSizes& z)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 256 * NEAT_RELU_BWD_WAVES));
+}
 int mfma_fwd_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 1024)); }
 
 size_t partials_bytes(const ProjSizes& z)
 {
-    size_t slices = std::max<size_t>((size_t)bw1_workgroups(z), (size_t)mfma_bwd_workgroups(z) * 4);
+    size_t slices = std::max<size_t>((size_t)bw1_workgroups(z), (size_t)mfma_bwd_relu_workgroups(z) * 4);
     return (slices * z.P * 4 + 255) / 256 * 256;
 }
 
@@ -1588,13 +1885,18 @@ int launch_bw_valu(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b
 template <typename P, int H>
 int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b, int stage)
 {
-    const int nwg  = mfma_bwd_workgroups(a.z);
-    const int Ptot = a.m.nparams();
+    // BF16 with relu hidden layers runs k_mfma_bwd_relu
+    const bool relu16 = P::kBF16 && a.act == NEAT_ACT_RELU;
+    const int nwg     = relu16 ? mfma_bwd_relu_workgroups(a.z) : mfma_bwd_workgroups(a.z);
+    const int Ptot    = a.m.nparams();
     if (stage == 1)
     {
         k_fill_f32<<<1, 64, 0, s>>>(reinterpret_cast<float*>(b.dmax), 1, 0.f);
         NEAT_LAUNCHED();
-        if (a.act == NEAT_ACT_RELU)
+        if (relu16)
+            k_mfma_bwd_relu<H><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, b.grad_proj, b.dfeat, b.sgc, b.dmax,
+                                                   b.partials, Ptot);
+        else if (a.act == NEAT_ACT_RELU)
             k_mfma_bwd<P, H, NEAT_ACT_RELU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, b.grad_proj, b.dfeat,
                                                                  b.sgc, b.dmax, b.partials, Ptot);
         else

@@ -127,3 +127,37 @@ def test_compensated_ray_sums_and_determinism(oracle, precision):
     e = rel(p1, ref[1][0])
     print(precision, "compensated projection error vs fp64 oracle", e)
     assert e <= (1e-5 if precision == "fp32" else 1e-3)
+
+
+@pytest.mark.parametrize("cfg_name,nrays", [("C1", 6000), ("C2", 640)])
+def test_bf16_relu_all_units_active(oracle, cfg_name, nrays):
+    """relu decoder whose hidden units are all active (biases far above the spread of the pre-activations): relu'
+    is then 1 everywhere in both FP64 and BF16, so no unit flips and the BF16 gradient error is the rounding error
+    alone, held to the smooth-activation bar (1e-2). Every weight, bias and feature gradient and every sample passes
+    through the BF16 products, so a wrong fragment index or operand orientation fails it by O(1)."""
+    print("case: bf16 relu all active", cfg_name, nrays, flush=True)
+    cfg = getattr(W, cfg_name)
+    sc, ds, o, d, to, td = scene(cfg, nrays, 23, act="relu")
+    h, c = cfg.hidden, cfg.channels
+    prm = sc["params"].copy()
+    o_b0 = h * c
+    o_w1 = o_b0 + h
+    o_b1 = o_w1 + h * h
+    o_w2 = o_b1 + h
+    prm[o_b0:o_b0 + h] += 2.0
+    prm[o_b1:o_b1 + h] += 12.0
+    prm[o_w2:o_w2 + h] *= 0.05
+    sc["params"] = prm
+    ds = pipeline.to_device(sc, cfg)
+    segs = pipeline.sample(ds, to, td, jitter=True, seed=9, emit=False)
+    gproj = np.random.default_rng(5).normal(size=o.shape[0]).astype(np.float32)
+    proj, gg, gp = fused(ds, to, td, segs, True, 9, torch.from_numpy(gproj).to(DEV), "bf16")
+    ref = oracle_both(oracle, sc, o, d, segs, True, 9, gproj)
+    e64 = [rel(proj, ref[1][0]), rel(gg, ref[1][1]), rel(gp, ref[1][2])]
+    # per parameter block, so that a block that is wrong cannot hide under the norm of a larger one
+    blocks = {"W0": (0, o_b0), "b0": (o_b0, o_w1), "W1": (o_w1, o_b1), "b1": (o_b1, o_w2), "W2": (o_w2, o_w2 + h),
+              "b2": (o_w2 + h, o_w2 + h + 1)}
+    eb = {k: rel(gp[a:b], ref[1][2][a:b]) for k, (a, b) in blocks.items()}
+    print("bf16 relu all-active errors vs fp64 oracle (proj, grid, params)", e64, "per block", eb)
+    assert e64[0] <= 1e-3
+    assert e64[1] <= 1e-2 and max(eb.values()) <= 1e-2
