@@ -29,6 +29,9 @@ HIP_FLAGS = [
     f"-I{os.path.join(ROOT, 'include')}", f"-I{KDIR}",
 ]
 
+# per-file additions: the relu BF16 backward keeps matrix-core results in VGPRs (its products feed vector ALU code)
+FILE_FLAGS = {"projector_relu.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
 
 def _run(cmd):
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
@@ -45,7 +48,7 @@ def build_abi(jobs=8):
 
     def compile_one(f):
         o = os.path.join(OBJ, f[:-4] + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(KDIR, f), "-o", o])
+        _run([HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(f, []), "-c", os.path.join(KDIR, f), "-o", o])
         return o
 
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -159,7 +159,7 @@ extern "C"
         else
             k_dec_bwd<64><<<nwg, kTile, 0, s>>>(x, S, m, wt, act, last_act, beta, gy, gx, partials);
         NEAT_LAUNCHED();
-        k_dec_reduce_partials<<<grid_for(P, 256), 256, 0, s>>>(partials, nwg, P, gparams);
+        k_dec_reduce_partials<<<reduce_partials_blocks(P), 256, 0, s>>>(partials, nwg, P, gparams);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }

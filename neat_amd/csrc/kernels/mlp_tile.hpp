@@ -255,14 +255,32 @@ static __global__ void __launch_bounds__(256) k_transpose_params(const float* pa
     }
 }
 
-// Fixed-order sum of the workgroup partials.
+// Fixed-order sum of num_wg partial slices [num_wg][P]: a block covers 32 parameters with 8 groups of 32 threads;
+// group q sums slices q, q + 8, q + 16, ... in order (loads of one thread are independent, so they overlap), then
+// the 8 group sums are added in group order. Deterministic for a given num_wg.
 static __global__ void __launch_bounds__(256) k_dec_reduce_partials(const float* partials, int num_wg, int P, float* out)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    float acc = 0.f;
-    for (int w = 0; w < num_wg; ++w) acc += partials[(int64_t)w * P + p];
-    out[p] = acc;
+    __shared__ float s[8][32];
+    const int lane = threadIdx.x & 31, q = threadIdx.x >> 5;
+    const int p    = blockIdx.x * 32 + lane;
+    float acc      = 0.f;
+    if (p < P)
+    {
+#pragma unroll 8
+        for (int w = q; w < num_wg; w += 8) acc += partials[(int64_t)w * P + p];
+    }
+    s[q][lane] = acc;
+    __syncthreads();
+    if (q == 0 && p < P)
+    {
+        float t = s[0][lane];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) t += s[k][lane];
+        out[p] = t;
+    }
 }
+
+// grid of k_dec_reduce_partials for P parameters (256 threads per block)
+static inline unsigned reduce_partials_blocks(int P) { return (unsigned)((P + 31) / 32); }
 
 }  // namespace neat

@@ -19,266 +19,10 @@
 
 #include <type_traits>
 
-#include "mfma_mlp.hpp"
-#include "mlp_tile.hpp"
+#include "projector_dev.hpp"
 
 namespace neat
 {
-
-constexpr int kFwdChunk   = 4096;  // samples per forward / backward-pass-1 chunk (and per forward window)
-constexpr int kSegCost    = 16;    // chunk cost of a segment, in samples
-constexpr int kSegCap     = kFwdChunk / kSegCost;  // segments per forward chunk, at most
-constexpr int kFwdThreads = 256;
-
-struct ProjGeom
-{
-    const float* pmin;
-    const float* pmax;
-    const int64_t* prefix;
-    int32_t num_nodes;
-    const float* ro;
-    const float* rd;
-    int64_t R;
-    const int64_t* ray_seg_offset;
-    int64_t K;
-    const int32_t* seg_node;
-    const int32_t* seg_ray;
-    const int32_t* seg_count;
-    const float* seg_tmin;
-    const float* seg_tmax;
-    int32_t jitter;
-    uint64_t seed;
-    const float* grid;  // channels-last [N, G, G, G, C]
-    int64_t N;
-    int32_t G;
-    float beta;
-    const float* seg_cdf;  // optional [K][8] proposal CDF (row used when cdf[7] > 0)
-    int64_t S;             // samples of the segment list
-};
-
-struct Plan
-{
-    int32_t* perm;         // [K] node-major position -> canonical segment index
-    int64_t* nm_soff;      // [K + 1] node-major sample offsets
-    int32_t* slot_begin;   // [N + 1] node-major segment range of each slot
-    int64_t* chunk_off1;   // [N + 1] first forward chunk of each slot
-    int64_t* chunk_off2;   // [N + 1] first scatter chunk of each slot
-    int32_t* chunk1_slot;  // [max1]
-    int32_t* chunk1_q0;
-    int32_t* chunk1_q1;
-    int32_t* chunk2_slot;  // [max2]
-    int32_t* chunk2_q0;
-    int32_t* chunk2_q1;
-    int64_t max1, max2;
-    int32_t chunk2;  // samples per scatter chunk
-};
-
-// ---- per-sample geometry (bit-identical to the unfused sampler + ComputeLocalSamples chain) -----------------------
-struct Sample
-{
-    float w;
-    float uv[3];
-    int64_t ray;
-};
-
-// Proposal-guided placement: n equal-probability cells of the piecewise-constant density over 8 equal bins whose CDF
-// is cdf (cell ends pinned to tmin / tmax); the sample sits at t1 (1 - a) + t2 a in its cell, weighted by the cell's
-// length. Same operations as the oracle's proposal_cell / sample_tw.
-__device__ __forceinline__ float inv_cdf8(const float (&cdf)[8], float u)
-{
-    int b    = 0;
-    float lo = 0.f, hi = cdf[0];
-#pragma unroll
-    for (int i = 0; i < 7; ++i)
-    {
-        if (b == i && cdf[i] <= u)
-        {
-            lo = cdf[i];
-            hi = cdf[i + 1];
-            b  = i + 1;
-        }
-    }
-    const float pb = hi - lo;
-    float f        = pb > 0.f ? (u - lo) / pb : 0.f;
-    f              = fminf(fmaxf(f, 0.f), 1.f);
-    return ((float)b + f) * 0.125f;
-}
-
-__device__ __forceinline__ float proposal_cell(const float (&cdf)[8], float tmin, float tmax, int32_t n, int32_t j)
-{
-    if (j <= 0) return tmin;
-    if (j >= n) return tmax;
-    return tmin + (tmax - tmin) * inv_cdf8(cdf, (float)j / (float)n);
-}
-
-// t and w of sample j of a segment: uniform (HyperTree.cu:339-389) or inside the proposal's cell j.
-__device__ __forceinline__ void sample_tw(const ProjGeom& g, bool prop, const float (&cdf)[8], float tmin, float tmax,
-                                          float step, int32_t n, int32_t j, int64_t ray, int32_t rank, float& t,
-                                          float& w)
-{
-    const float a = seg_a(g.jitter, g.seed, ray, rank, j);
-    if (!prop)
-    {
-        t = sample_t(tmin, step, j, a);
-        w = sample_w(tmin, tmax, step, n, j, t, g.jitter, g.seed, ray, rank);
-        return;
-    }
-    const float t1 = proposal_cell(cdf, tmin, tmax, n, j);
-    const float t2 = proposal_cell(cdf, tmin, tmax, n, j + 1);
-    t              = t1 * (1.0f - a) + t2 * a;
-    w              = t2 - t1;
-}
-
-__device__ __forceinline__ bool load_cdf(const ProjGeom& g, int64_t k, float (&cdf)[8])
-{
-    if (!g.seg_cdf) return false;
-    const float4* r = reinterpret_cast<const float4*>(g.seg_cdf + 8 * k);
-    const float4 u = r[0], v = r[1];
-    cdf[0] = u.x; cdf[1] = u.y; cdf[2] = u.z; cdf[3] = u.w;
-    cdf[4] = v.x; cdf[5] = v.y; cdf[6] = v.z; cdf[7] = v.w;
-    return cdf[7] > 0.f;
-}
-
-__device__ __forceinline__ Sample sample_geo(const ProjGeom& g, int32_t k, int32_t j)
-{
-    Sample s;
-    const int64_t ray  = g.seg_ray[k];
-    const int32_t n    = g.seg_count[k];
-    const int32_t rank = (int32_t)(k - g.ray_seg_offset[ray]);
-    const int32_t node = g.seg_node[k];
-    const float tmin = g.seg_tmin[k], tmax = g.seg_tmax[k];
-    const float step = (tmax - tmin) / (float)n;
-    float cdf[8];
-    const bool prop = load_cdf(g, k, cdf);
-    float t;
-    sample_tw(g, prop, cdf, tmin, tmax, step, n, j, ray, rank, t, s.w);
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-    {
-        float p   = g.ro[3 * ray + d] + g.rd[3 * ray + d] * t;
-        float mn  = g.pmin[3 * node + d], mx = g.pmax[3 * node + d];
-        s.uv[d]   = (p - mn) / (mx - mn) * 2 - 1;
-    }
-    s.ray = ray;
-    return s;
-}
-
-// Trilinear corners in a cubic channels-last node grid (offsets in floats from the node's first voxel).
-struct Corners
-{
-    float w[8];
-    int32_t off[8];
-};
-
-__device__ __forceinline__ int clamp_g(int v, int G) { return min(G - 1, max(v, 0)); }
-
-// Continuous grid coordinate of a local coordinate (align_corners, IndirectGridSample3D.cu:448-455).
-__device__ __forceinline__ float grid_coord(float u, int G) { return ((u + 1) / 2) * (float)(G - 1); }
-
-__device__ __forceinline__ Corners corners_ix(float ix, float iy, float iz, int G, int C)
-{
-    Corners c;
-    int x0 = (int)floorf(ix), y0 = (int)floorf(iy), z0 = (int)floorf(iz);
-    int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
-    c.w[0] = ((float)x1 - ix) * ((float)y1 - iy) * ((float)z1 - iz);
-    c.w[1] = (ix - (float)x0) * ((float)y1 - iy) * ((float)z1 - iz);
-    c.w[2] = ((float)x1 - ix) * (iy - (float)y0) * ((float)z1 - iz);
-    c.w[3] = (ix - (float)x0) * (iy - (float)y0) * ((float)z1 - iz);
-    c.w[4] = ((float)x1 - ix) * ((float)y1 - iy) * (iz - (float)z0);
-    c.w[5] = (ix - (float)x0) * ((float)y1 - iy) * (iz - (float)z0);
-    c.w[6] = ((float)x1 - ix) * (iy - (float)y0) * (iz - (float)z0);
-    c.w[7] = (ix - (float)x0) * (iy - (float)y0) * (iz - (float)z0);
-    const int cx0 = clamp_g(x0, G), cx1 = clamp_g(x1, G);
-    const int cy0 = clamp_g(y0, G), cy1 = clamp_g(y1, G);
-    const int cz0 = clamp_g(z0, G), cz1 = clamp_g(z1, G);
-    const int zs = G * G * C, ys = G * C;
-    c.off[0] = cz0 * zs + cy0 * ys + cx0 * C;
-    c.off[1] = cz0 * zs + cy0 * ys + cx1 * C;
-    c.off[2] = cz0 * zs + cy1 * ys + cx0 * C;
-    c.off[3] = cz0 * zs + cy1 * ys + cx1 * C;
-    c.off[4] = cz1 * zs + cy0 * ys + cx0 * C;
-    c.off[5] = cz1 * zs + cy0 * ys + cx1 * C;
-    c.off[6] = cz1 * zs + cy1 * ys + cx0 * C;
-    c.off[7] = cz1 * zs + cy1 * ys + cx1 * C;
-    return c;
-}
-
-__device__ __forceinline__ Corners corners_cl(const float uv[3], int G, int C)
-{
-    return corners_ix(grid_coord(uv[0], G), grid_coord(uv[1], G), grid_coord(uv[2], G), G, C);
-}
-
-template <int C>
-__device__ __forceinline__ void gather_cl(const float* node_grid, const Corners& c, float (&x)[C])
-{
-#pragma unroll
-    for (int q = 0; q < C; q += 4)
-    {
-        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-        {
-            float4 v = *reinterpret_cast<const float4*>(node_grid + c.off[k] + q);
-            acc.x += v.x * c.w[k];
-            acc.y += v.y * c.w[k];
-            acc.z += v.z * c.w[k];
-            acc.w += v.w * c.w[k];
-        }
-        x[q]     = acc.x;
-        x[q + 1] = acc.y;
-        x[q + 2] = acc.z;
-        x[q + 3] = acc.w;
-    }
-}
-
-// Register-resident decoder C -> H -> H -> 1 (one hidden layer), packed parameters read with uniform loads.
-// Returns the output pre-activation.
-template <int C, int H, int ACT>
-__device__ __forceinline__ float mlp_forward_reg(const float* __restrict__ P, const float (&x)[C], float beta)
-{
-    const float* W0 = P;
-    const float* b0 = W0 + H * C;
-    const float* W1 = b0 + H;
-    const float* b1 = W1 + H * H;
-    const float* W2 = b1 + H;
-    const float* b2 = W2 + H;
-    float h0[H];
-#pragma unroll
-    for (int o = 0; o < H; ++o)
-    {
-        float acc = 0.f;
-#pragma unroll
-        for (int k = 0; k < C; ++k) acc = fmaf(W0[o * C + k], x[k], acc);
-        h0[o] = act_f(ACT, acc + b0[o], beta);
-    }
-    float acc2 = 0.f;
-#pragma unroll
-    for (int o = 0; o < H; ++o)
-    {
-        float acc = 0.f;
-#pragma unroll
-        for (int k = 0; k < H; ++k) acc = fmaf(W1[o * H + k], h0[k], acc);
-        float h1 = act_f(ACT, acc + b1[o], beta);
-        acc2     = fmaf(W2[o], h1, acc2);
-    }
-    return acc2 + b2[0];
-}
-
-// Node-major position of the segment holding chunk-relative sample s: the last q in [q0, q1) with
-// nm_soff[q] - base <= s.
-__device__ __forceinline__ int32_t find_segment(const int64_t* nm_soff, int32_t q0, int32_t q1, int64_t target)
-{
-    int32_t lo = q0, hi = q1 - 1;
-    while (lo < hi)
-    {
-        int32_t mid = (lo + hi + 1) >> 1;
-        if (nm_soff[mid] <= target)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
 
 // ---- plan ---------------------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_plan_keys(ProjGeom g, int32_t* keys, int32_t* vals)
@@ -782,219 +526,6 @@ __global__ void __launch_bounds__(256) k_fill_f32(float* p, int64_t n, float v)
     if (i < n) p[i] = v;
 }
 
-// ---- matrix-core path ---------------------------------------------------------------------------------------------------
-// Gather of the lane's 4 channels (group g: channels 4g..4g+3) of an 8-channel node grid, FP32 or BF16 storage,
-// trilinear weights and accumulation in FP32 (same corner order as the reference).
-template <bool kBF16Grid>
-__device__ __forceinline__ void gather_half(const void* node_grid, const Corners& c, int g, float (&x4)[4])
-{
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-    {
-        float4 v;
-        if constexpr (kBF16Grid)
-        {
-            uint2 raw = *reinterpret_cast<const uint2*>(static_cast<const unsigned short*>(node_grid) + c.off[k] + 4 * g);
-            v = make_float4(__uint_as_float(raw.x << 16), __uint_as_float(raw.x & 0xFFFF0000u),
-                            __uint_as_float(raw.y << 16), __uint_as_float(raw.y & 0xFFFF0000u));
-        }
-        else
-        {
-            v = *reinterpret_cast<const float4*>(static_cast<const float*>(node_grid) + c.off[k] + 4 * g);
-        }
-        acc.x += v.x * c.w[k];
-        acc.y += v.y * c.w[k];
-        acc.z += v.z * c.w[k];
-        acc.w += v.w * c.w[k];
-    }
-    x4[0] = acc.x;
-    x4[1] = acc.y;
-    x4[2] = acc.z;
-    x4[3] = acc.w;
-}
-
-__device__ __forceinline__ const void* node_grid_ptr(const ProjGeom& g, const void* grid_lowp, bool bf16, int32_t slot)
-{
-    const int64_t off = (int64_t)slot * g.G * g.G * g.G * 8;
-    return bf16 ? static_cast<const void*>(static_cast<const unsigned short*>(grid_lowp) + off)
-                : static_cast<const void*>(g.grid + off);
-}
-
-// Segment state of one forward chunk, staged in LDS once per chunk: what the chunk's samples need from the segment
-// list and the rays, so that a sample's geometry costs LDS reads rather than chains of dependent global loads.
-struct ChunkCtx
-{
-    int32_t beg[kSegCap + 1];  // chunk-relative first sample of each segment; beg[nseg] = the chunk's sample count
-    int32_t k[kSegCap];        // canonical segment index
-    int32_t ray[kSegCap];
-    int32_t rank[kSegCap];
-    int32_t n[kSegCap];
-    float tmin[kSegCap], tmax[kSegCap], step[kSegCap];
-    float gproj[kSegCap];  // dL/dprojection of the segment's ray (backward)
-    float o[3][kSegCap], d[3][kSegCap];
-    float cdf[8][kSegCap];  // proposal CDF rows (cdf[7][t] == 0: uniform)
-    uint32_t hray[2][kSegCap];  // jitter_ray_hash of the segment's ray (low, high word)
-    uint16_t sq[kFwdChunk];  // window-relative sample -> chunk-relative segment
-};
-
-__device__ void load_chunk_ctx(const ProjGeom& g, const Plan& p, int32_t q0, int32_t q1, int64_t base,
-                               const float* grad_proj, ChunkCtx& c)
-{
-    const int32_t nseg = q1 - q0;
-    for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x)
-    {
-        const int32_t q   = q0 + t;
-        const int32_t k   = p.perm[q];
-        const int32_t ray = g.seg_ray[k];
-        const int32_t n   = g.seg_count[k];
-        const float tmin = g.seg_tmin[k], tmax = g.seg_tmax[k];
-        c.beg[t] = (int32_t)(p.nm_soff[q] - base);
-        if (t == nseg - 1) c.beg[nseg] = (int32_t)(p.nm_soff[q1] - base);
-        c.k[t]     = k;
-        c.ray[t]   = ray;
-        c.rank[t]  = (int32_t)(k - g.ray_seg_offset[ray]);
-        c.n[t]     = n;
-        c.tmin[t]  = tmin;
-        c.tmax[t]  = tmax;
-        c.step[t]  = (tmax - tmin) / (float)n;
-        c.gproj[t] = grad_proj ? grad_proj[ray] : 0.f;
-        const uint64_t hr = jitter_ray_hash(g.seed, ray);
-        c.hray[0][t]      = (uint32_t)hr;
-        c.hray[1][t]      = (uint32_t)(hr >> 32);
-        float cdf[8];
-        const bool prop = load_cdf(g, k, cdf);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) c.cdf[b][t] = prop ? cdf[b] : 0.f;
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-        {
-            c.o[d][t] = g.ro[3 * (int64_t)ray + d];
-            c.d[d][t] = g.rd[3 * (int64_t)ray + d];
-        }
-    }
-}
-
-// sq for the window [w0, w0 + wn) of the chunk's samples (needs beg[] complete).
-__device__ void expand_window(ChunkCtx& c, int32_t nseg, int32_t w0, int32_t wn)
-{
-    for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x)
-    {
-        const int32_t a = max(c.beg[t], w0), b = min(c.beg[t + 1], w0 + wn);
-        for (int32_t i = a; i < b; ++i) c.sq[i - w0] = (uint16_t)t;
-    }
-}
-
-// Sample j of chunk segment t: the same operations as sample_geo on the same values, so the same bits.
-__device__ __forceinline__ Sample ctx_sample(const ProjGeom& g, const ChunkCtx& c, int32_t t, int32_t j,
-                                             const float (&mn)[3], const float (&mx)[3])
-{
-    Sample s;
-    const int64_t ray  = c.ray[t];
-    const int32_t rank = c.rank[t], n = c.n[t];
-    const float tmin = c.tmin[t], tmax = c.tmax[t], step = c.step[t];
-    const uint64_t hr = ((uint64_t)c.hray[1][t] << 32) | c.hray[0][t];
-    auto a_of         = [&](int32_t jj) { return g.jitter ? jitter_a_h(hr, rank, jj) : 0.5f; };
-    float tt;
-    if (c.cdf[7][t] > 0.f)
-    {
-        float cdf[8];
-#pragma unroll
-        for (int b = 0; b < 8; ++b) cdf[b] = c.cdf[b][t];
-        const float t1 = proposal_cell(cdf, tmin, tmax, n, j);
-        const float t2 = proposal_cell(cdf, tmin, tmax, n, j + 1);
-        const float a  = a_of(j);
-        tt             = t1 * (1.0f - a) + t2 * a;
-        s.w            = t2 - t1;
-    }
-    else
-    {
-        // sample_t / sample_w with the ray's half of the hash taken from the context
-        tt             = sample_t(tmin, step, j, a_of(j));
-        const float lo = (j == 0) ? tmin : (sample_t(tmin, step, j - 1, a_of(j - 1)) + tt) * 0.5f;
-        const float hi = (j == n - 1) ? tmax : (sample_t(tmin, step, j + 1, a_of(j + 1)) + tt) * 0.5f;
-        s.w            = hi - lo;
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-    {
-        float pd = c.o[d][t] + c.d[d][t] * tt;
-        s.uv[d]  = (pd - mn[d]) / (mx[d] - mn[d]) * 2 - 1;
-    }
-    s.ray = ray;
-    return s;
-}
-
-__device__ __forceinline__ void node_bounds(const ProjGeom& g, int32_t node, float (&mn)[3], float (&mx)[3])
-{
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-    {
-        mn[d] = g.pmin[3 * node + d];
-        mx[d] = g.pmax[3 * node + d];
-    }
-}
-
-// Inputs of a pair of 32-sample tiles (samples p0 .. p0 + 63 of the window): every lane computes the geometry of one
-// sample (lane l: sample p0 + l) and the lane groups swap their corners, so that each lane holds the corners of
-// sample p0 + n (tile A) and p0 + 32 + n (tile B) and gathers its 4 channels of both. v is the sample weight
-// (forward) or dL/dy = dL/dproj x weight (backward); samples past the window have zero weights, features and v.
-struct PairIn
-{
-    float xa[4], xb[4];
-    float va, vb;
-};
-
-template <bool kBF16, bool kBwd>
-__device__ __forceinline__ void fetch_pair(const ProjGeom& g, const ChunkCtx& cx, int32_t p0, int32_t w0,
-                                           int32_t wn, int64_t base, const float (&mn)[3], const float (&mx)[3],
-                                           const void* ngrid, float4* sgc, PairIn& in)
-{
-    const int l = threadIdx.x & 63, hg = l >> 5;
-    const int32_t ii = p0 + l;
-    Corners cr;
-    float v = 0.f;
-    if (ii < wn)
-    {
-        const int32_t t = cx.sq[ii];
-        Sample smp      = ctx_sample(g, cx, t, w0 + ii - cx.beg[t], mn, mx);
-        const float ix = grid_coord(smp.uv[0], g.G), iy = grid_coord(smp.uv[1], g.G), iz = grid_coord(smp.uv[2], g.G);
-        cr = corners_ix(ix, iy, iz, g.G, 8);
-        v  = kBwd ? cx.gproj[t] * smp.w : smp.w;
-        if (kBwd) sgc[base + w0 + ii] = make_float4(ix, iy, iz, 0.f);
-    }
-    else
-    {
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-        {
-            cr.off[k] = 0;
-            cr.w[k]   = 0.f;
-        }
-    }
-    Corners pc;  // the other lane group's sample
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-    {
-        pc.off[k] = __shfl_xor(cr.off[k], 32);
-        pc.w[k]   = __shfl_xor(cr.w[k], 32);
-    }
-    const float pv = __shfl_xor(v, 32);
-    Corners ca, cb;
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-    {
-        ca.off[k] = hg == 0 ? cr.off[k] : pc.off[k];
-        ca.w[k]   = hg == 0 ? cr.w[k] : pc.w[k];
-        cb.off[k] = hg == 1 ? cr.off[k] : pc.off[k];
-        cb.w[k]   = hg == 1 ? cr.w[k] : pc.w[k];
-    }
-    gather_half<kBF16>(ngrid, ca, hg, in.xa);
-    gather_half<kBF16>(ngrid, cb, hg, in.xb);
-    in.va = hg == 0 ? v : pv;
-    in.vb = hg == 1 ? v : pv;
-}
-
 // Forward: persistent workgroups of 4 waves over the chunks; tiles of 32 samples per wave on the matrix cores;
 // per-segment sums sequential in j as in k_proj_fwd.
 template <typename P, int H, int ACT>
@@ -1315,298 +846,6 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
     if (l == 0) atomicMax(dmax, __float_as_uint(amax));  // non-negative floats order like their bit patterns
 }
 
-// Backward for BF16 relu decoders (the configuration the projector benchmarks): as k_mfma_bwd, but every product is
-// formed on the matrix cores in the orientation its consumer needs (WeightLDSRelu), so no tile is transposed
-// through LDS. Per wave and tile of 32 samples, with "h" tiles holding hidden units in the registers and samples on
-// the lanes, and "s" tiles the transpose:
-//   forward  Z1h, Z1s = W0 [x;1]; H1 = relu(Z1) (BF16); Z2h, Z2s = W1 H1 + b1; z = w2 . relu(Z2h) + b2
-//   backward dz = dL/dz; D2 = dz * [Z2 > 0] (h and s; the output weights are folded into w2t and the final scale)
-//            dW1 += D2s H1s^T; E += D2s [e10] + relu(Z2s) [dz e9] + D1s [x; 1]  (bias / output / first-layer grads)
-//            D1h, D1s = (w2t D2h) * [H1 > 0]; dL/dfeature = W0^T D1h
-// The relu masks come from the sign of the BF16 activations, so only BF16 fragments stay live across the tile.
-typedef short s16x8 __attribute__((ext_vector_type(8)));
-
-// waves per SIMD the relu backward is compiled for (its register budget: 512 / waves per lane)
-#ifndef NEAT_RELU_BWD_WAVES
-#define NEAT_RELU_BWD_WAVES 1
-#endif
-
-__device__ __forceinline__ bf16x8 relu_frag(const f32x16& z, int st)
-{
-    bf16x8 f;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) f[j] = (__bf16)fmaxf(z[8 * st + j], 0.f);
-    return f;
-}
-
-// BF16 fragment of z (slots 8st..8st+7) where the relu activation h > 0, else 0
-__device__ __forceinline__ bf16x8 masked_frag(const f32x16& z, int st, const bf16x8& h)
-{
-    bf16x8 f;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) f[j] = (__bf16)z[8 * st + j];
-    const s16x8 hb = __builtin_bit_cast(s16x8, h);
-    const s16x8 fb = __builtin_bit_cast(s16x8, f) & (hb > (short)0);
-    return __builtin_bit_cast(bf16x8, fb);
-}
-
-template <int H>
-__global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(ProjGeom g, Plan p, const float* __restrict__ params,
-                                                          const void* grid_lowp, const float* grad_proj, float* dfeat,
-                                                          float4* sgc, unsigned int* dmax, float* partials,
-                                                          int32_t P_total)
-{
-    constexpr int MT = H / 32, C = 8;
-    typedef PrecBF16 P;
-    __shared__ WeightLDSRelu<H> w;
-    __shared__ ChunkCtx cx;
-    __shared__ __align__(16) float s_dz[4][32];
-    load_weights_relu<H>(params, w);
-    __syncthreads();  // the final partial write reads w.w3 even in a workgroup that gets no chunk
-    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
-    // constant fragments: ONES (group 0 slots 0, 1) against the hidden bias; IDB routes the slots of the feature
-    // fragment to columns of the s-tile T (channels 0..7, 1 -> 8, 1 -> 10, dz -> 9)
-    bf16x8 ones = {}, idb = {};
-    if (hg == 0)
-    {
-        ones[0] = (__bf16)1.f;
-        ones[1] = (__bf16)1.f;
-    }
-    {
-        const int col[2][8] = {{0, 1, 2, 3, 8, 10, 9, -1}, {4, 5, 6, 7, -1, -1, -1, -1}};
-#pragma unroll
-        for (int j = 0; j < 8; ++j) idb[j] = (__bf16)(col[hg][j] == n ? 1.f : 0.f);
-    }
-    f32x16 dW2[MT][MT], E[MT];
-#pragma unroll
-    for (int a = 0; a < MT; ++a)
-    {
-        E[a] = f32x16{};
-#pragma unroll
-        for (int b = 0; b < MT; ++b) dW2[a][b] = f32x16{};
-    }
-    float b3 = 0.f;
-    float amax = 0.f;
-    float* sdz = s_dz[wave];
-    const int64_t nchunks = p.chunk_off1[g.N];
-    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x)
-    {
-        const int32_t slot = p.chunk1_slot[c];
-        const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c], nseg = q1 - q0;
-        if (nseg == 0) continue;
-        const int64_t base = p.nm_soff[q0];
-        __syncthreads();
-        load_chunk_ctx(g, p, q0, q1, base, grad_proj, cx);
-        float mn[3], mx[3];
-        node_bounds(g, g.seg_node[p.perm[q0]], mn, mx);
-        const void* ngrid = node_grid_ptr(g, grid_lowp, true, slot);
-        __syncthreads();
-        const int32_t nsamp = cx.beg[nseg];
-        for (int32_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
-        {
-            const int32_t wn = min(kFwdChunk, nsamp - w0);
-            expand_window(cx, nseg, w0, wn);
-            __syncthreads();
-            auto tile = [&](int32_t i, const float (&x4)[4], float gy) {
-                const bool valid = i < wn;
-                // feature fragment: channels 4hg..4hg+3, group 0 adds the two bias slots (1, 1)
-                bf16x8 X = {};
-#pragma unroll
-                for (int j = 0; j < 4; ++j) X[j] = (__bf16)x4[j];
-                if (hg == 0)
-                {
-                    X[4] = (__bf16)1.f;
-                    X[5] = (__bf16)1.f;
-                }
-                // first layer, both orientations
-                bf16x8 H1h[MT][2], H1s[MT][2];
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                {
-                    const f32x16 zh = P::mma(w.w1e[mt][l], X, f32x16{});
-                    const f32x16 zs = P::mma(X, w.w1e[mt][l], f32x16{});
-#pragma unroll
-                    for (int st = 0; st < 2; ++st)
-                    {
-                        H1h[mt][st] = relu_frag(zh, st);
-                        H1s[mt][st] = relu_frag(zs, st);
-                    }
-                }
-                // hidden layer (h): output pre-activation and dz
-                bf16x8 D2h[MT][2];
-                float dz;
-                {
-                    f32x16 Z2h[MT];
-                    float part = 0.f;
-#pragma unroll
-                    for (int mt = 0; mt < MT; ++mt)
-                    {
-                        f32x16 acc = P::mma(w.w2b[mt][l], ones, f32x16{});
-#pragma unroll
-                        for (int ms = 0; ms < MT; ++ms)
-                        {
-#pragma unroll
-                            for (int st = 0; st < 2; ++st) acc = P::mma(w.w2[mt][ms][st][l], H1h[ms][st], acc);
-                        }
-                        Z2h[mt] = acc;
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                        {
-                            const float4 w3 = *reinterpret_cast<const float4*>(&w.w3[32 * mt + 8 * q + 4 * hg]);
-                            part = fmaf(w3.x, fmaxf(acc[4 * q + 0], 0.f), part);
-                            part = fmaf(w3.y, fmaxf(acc[4 * q + 1], 0.f), part);
-                            part = fmaf(w3.z, fmaxf(acc[4 * q + 2], 0.f), part);
-                            part = fmaf(w3.w, fmaxf(acc[4 * q + 3], 0.f), part);
-                        }
-                    }
-                    const float z = part + __shfl_xor(part, 32) + w.b3;
-                    dz = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, z, g.beta) : 0.f;
-#pragma unroll
-                    for (int mt = 0; mt < MT; ++mt)
-                    {
-#pragma unroll
-                        for (int st = 0; st < 2; ++st)
-                        {
-#pragma unroll
-                            for (int j = 0; j < 8; ++j) D2h[mt][st][j] = (__bf16)(Z2h[mt][8 * st + j] > 0.f ? dz : 0.f);
-                        }
-                    }
-                }
-                if (hg == 0) b3 += dz;
-                // dz per register of an s tile (sample crow(i, hg)), through LDS
-                wave_lds_fence();
-                if (hg == 0) sdz[n] = dz;
-                wave_lds_fence();
-                float dzs[16];
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                {
-                    const float4 v = *reinterpret_cast<const float4*>(&sdz[8 * q + 4 * hg]);
-                    dzs[4 * q + 0] = v.x;
-                    dzs[4 * q + 1] = v.y;
-                    dzs[4 * q + 2] = v.z;
-                    dzs[4 * q + 3] = v.w;
-                }
-                // the s tile T of [x; 1; dz; 1] (columns 0..7, 8, 9, 10) and its masked B fragments
-                bf16x8 B08[2], B9[2], B10[2];
-                {
-                    if (hg == 0) X[6] = (__bf16)dz;
-                    const f32x16 T = P::mma(X, idb, f32x16{});
-#pragma unroll
-                    for (int st = 0; st < 2; ++st)
-                    {
-                        bf16x8 t;
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) t[j] = (__bf16)T[8 * st + j];
-                        B08[st] = n <= 8 ? t : bf16x8{};
-                        B9[st]  = n == 9 ? t : bf16x8{};
-                        B10[st] = n == 10 ? t : bf16x8{};
-                    }
-                }
-                // hidden layer (s): relu(Z2s) and D2s; output-weight and hidden-bias gradients; hidden-weight gradient
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                {
-                    f32x16 acc = P::mma(ones, w.w2b[mt][l], f32x16{});
-#pragma unroll
-                    for (int ms = 0; ms < MT; ++ms)
-                    {
-#pragma unroll
-                        for (int st = 0; st < 2; ++st) acc = P::mma(H1h[ms][st], w.w2[mt][ms][st][l], acc);
-                    }
-#pragma unroll
-                    for (int st = 0; st < 2; ++st)
-                    {
-                        bf16x8 h2 = relu_frag(acc, st), d2;
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) d2[j] = (__bf16)(acc[8 * st + j] > 0.f ? dzs[8 * st + j] : 0.f);
-                        E[mt] = P::mma(h2, B9[st], E[mt]);
-                        E[mt] = P::mma(d2, B10[st], E[mt]);
-#pragma unroll
-                        for (int m1 = 0; m1 < MT; ++m1) dW2[mt][m1] = P::mma(d2, H1s[m1][st], dW2[mt][m1]);
-                    }
-                }
-                // first layer (s): D1s and the first-layer weight / bias gradients
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                {
-                    f32x16 acc = {};
-#pragma unroll
-                    for (int ms = 0; ms < MT; ++ms)
-                    {
-#pragma unroll
-                        for (int st = 0; st < 2; ++st) acc = P::mma(D2h[ms][st], w.w2t[mt][ms][st][l], acc);
-                    }
-#pragma unroll
-                    for (int st = 0; st < 2; ++st) E[mt] = P::mma(masked_frag(acc, st, H1s[mt][st]), B08[st], E[mt]);
-                }
-                // first layer (h): D1h and dL/dfeature = W0^T D1h (rows 0..7: registers 0..3 of each lane group)
-                f32x16 dx = {};
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                {
-                    f32x16 acc = {};
-#pragma unroll
-                    for (int ms = 0; ms < MT; ++ms)
-                    {
-#pragma unroll
-                        for (int st = 0; st < 2; ++st) acc = P::mma(w.w2t[mt][ms][st][l], D2h[ms][st], acc);
-                    }
-#pragma unroll
-                    for (int st = 0; st < 2; ++st) dx = P::mma(w.w1t[mt][st][l], masked_frag(acc, st, H1h[mt][st]), dx);
-                }
-                if (valid)
-                {
-                    *reinterpret_cast<float4*>(dfeat + ((int64_t)hg * g.S + base + w0 + i) * 4) =
-                        make_float4(dx[0], dx[1], dx[2], dx[3]);
-                    amax = fmaxf(amax, fmaxf(fmaxf(fabsf(dx[0]), fabsf(dx[1])), fmaxf(fabsf(dx[2]), fabsf(dx[3]))));
-                }
-            };
-            for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
-            {
-                PairIn cur;
-                fetch_pair<true, true>(g, cx, p0, w0, wn, base, mn, mx, ngrid, sgc, cur);
-#pragma unroll 1
-                for (int h = 0; h < 2; ++h)
-                {
-                    if (h == 1 && p0 + 32 >= wn) break;
-                    float xs[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) xs[j] = h ? cur.xb[j] : cur.xa[j];
-                    tile(p0 + 32 * h + n, xs, h ? cur.vb : cur.va);
-                }
-            }
-            __syncthreads();
-        }
-    }
-    // ---- this wave's slice of the partial parameter gradient (packed layout); rows scaled by the output weights
-    // where the output weights were folded out of D2 ----
-    float* part   = partials + ((int64_t)blockIdx.x * 4 + wave) * P_total;
-    const int oW0 = 0, ob0 = H * C, oW1 = ob0 + H, ob1 = oW1 + H * H, oW2 = ob1 + H, ob2 = oW2 + H;
-#pragma unroll
-    for (int m2 = 0; m2 < MT; ++m2)
-    {
-#pragma unroll
-        for (int i2 = 0; i2 < 16; ++i2)
-        {
-            const int r    = 32 * m2 + crow(i2, hg);
-            const float s3 = w.w3[r];
-#pragma unroll
-            for (int m1 = 0; m1 < MT; ++m1) part[oW1 + r * H + 32 * m1 + n] = dW2[m2][m1][i2] * s3;
-            if (n < C) part[oW0 + r * C + n] = E[m2][i2];
-            if (n == 8) part[ob0 + r] = E[m2][i2];
-            if (n == 9) part[oW2 + r] = E[m2][i2];
-            if (n == 10) part[ob1 + r] = E[m2][i2] * s3;
-        }
-    }
-#pragma unroll
-    for (int off = 16; off >= 1; off >>= 1) b3 += __shfl_xor(b3, off);
-    if (l == 0) part[ob2] = b3;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off));
-    if (l == 0) atomicMax(dmax, __float_as_uint(amax));
-}
-
 // ---- workspace layout ---------------------------------------------------------------------------------------------------
 struct Layout
 {
@@ -1875,7 +1114,7 @@ int launch_bw_valu(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b
     }
     if (stage == 2)
     {
-        k_dec_reduce_partials<<<grid_for(P, 256), 256, 0, s>>>(b.partials, nwg, P, b.grad_params);
+        k_dec_reduce_partials<<<reduce_partials_blocks(P), 256, 0, s>>>(b.partials, nwg, P, b.grad_params);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }
@@ -1894,9 +1133,13 @@ int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b
         k_fill_f32<<<1, 64, 0, s>>>(reinterpret_cast<float*>(b.dmax), 1, 0.f);
         NEAT_LAUNCHED();
         if (relu16)
-            k_mfma_bwd_relu<H><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, b.grad_proj, b.dfeat, b.sgc, b.dmax,
-                                                   b.partials, Ptot);
-        else if (a.act == NEAT_ACT_RELU)
+        {
+            NEAT_REQUIRE(launch_mfma_bwd_relu(s, H, (unsigned)nwg, a.g, p, a.params, a.grid_bf16, b.grad_proj, b.dfeat,
+                                              b.sgc, b.dmax, b.partials, Ptot) == 0,
+                         "k_mfma_bwd_relu launch failed");
+            return NEAT_OK;
+        }
+        if (a.act == NEAT_ACT_RELU)
             k_mfma_bwd<P, H, NEAT_ACT_RELU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, b.grad_proj, b.dfeat,
                                                                  b.sgc, b.dmax, b.partials, Ptot);
         else
@@ -1907,7 +1150,7 @@ int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b
     }
     if (stage == 2)
     {
-        k_dec_reduce_partials<<<grid_for(Ptot, 256), 256, 0, s>>>(b.partials, nwg * 4, Ptot, b.grad_params);
+        k_dec_reduce_partials<<<reduce_partials_blocks(Ptot), 256, 0, s>>>(b.partials, nwg * 4, Ptot, b.grad_params);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }

@@ -1,0 +1,492 @@
+// Device-side pieces shared by the projector's kernels (projector.hip, projector_relu.hip): the argument structs,
+// per-sample geometry, corner gathers, and the chunk context and pair fetch of the matrix-core kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mfma_mlp.hpp"
+#include "mlp_tile.hpp"
+
+namespace neat
+{
+
+constexpr int kFwdChunk   = 4096;  // samples per forward / backward-pass-1 chunk (and per forward window)
+constexpr int kSegCost    = 16;    // chunk cost of a segment, in samples
+constexpr int kSegCap     = kFwdChunk / kSegCost;  // segments per forward chunk, at most
+constexpr int kFwdThreads = 256;
+
+struct ProjGeom
+{
+    const float* pmin;
+    const float* pmax;
+    const int64_t* prefix;
+    int32_t num_nodes;
+    const float* ro;
+    const float* rd;
+    int64_t R;
+    const int64_t* ray_seg_offset;
+    int64_t K;
+    const int32_t* seg_node;
+    const int32_t* seg_ray;
+    const int32_t* seg_count;
+    const float* seg_tmin;
+    const float* seg_tmax;
+    int32_t jitter;
+    uint64_t seed;
+    const float* grid;  // channels-last [N, G, G, G, C]
+    int64_t N;
+    int32_t G;
+    float beta;
+    const float* seg_cdf;  // optional [K][8] proposal CDF (row used when cdf[7] > 0)
+    int64_t S;             // samples of the segment list
+};
+
+struct Plan
+{
+    int32_t* perm;         // [K] node-major position -> canonical segment index
+    int64_t* nm_soff;      // [K + 1] node-major sample offsets
+    int32_t* slot_begin;   // [N + 1] node-major segment range of each slot
+    int64_t* chunk_off1;   // [N + 1] first forward chunk of each slot
+    int64_t* chunk_off2;   // [N + 1] first scatter chunk of each slot
+    int32_t* chunk1_slot;  // [max1]
+    int32_t* chunk1_q0;
+    int32_t* chunk1_q1;
+    int32_t* chunk2_slot;  // [max2]
+    int32_t* chunk2_q0;
+    int32_t* chunk2_q1;
+    int64_t max1, max2;
+    int32_t chunk2;  // samples per scatter chunk
+};
+
+// ---- per-sample geometry (bit-identical to the unfused sampler + ComputeLocalSamples chain) -----------------------
+struct Sample
+{
+    float w;
+    float uv[3];
+    int64_t ray;
+};
+
+// Proposal-guided placement: n equal-probability cells of the piecewise-constant density over 8 equal bins whose CDF
+// is cdf (cell ends pinned to tmin / tmax); the sample sits at t1 (1 - a) + t2 a in its cell, weighted by the cell's
+// length. Same operations as the oracle's proposal_cell / sample_tw.
+__device__ __forceinline__ float inv_cdf8(const float (&cdf)[8], float u)
+{
+    int b    = 0;
+    float lo = 0.f, hi = cdf[0];
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+    {
+        if (b == i && cdf[i] <= u)
+        {
+            lo = cdf[i];
+            hi = cdf[i + 1];
+            b  = i + 1;
+        }
+    }
+    const float pb = hi - lo;
+    float f        = pb > 0.f ? (u - lo) / pb : 0.f;
+    f              = fminf(fmaxf(f, 0.f), 1.f);
+    return ((float)b + f) * 0.125f;
+}
+
+__device__ __forceinline__ float proposal_cell(const float (&cdf)[8], float tmin, float tmax, int32_t n, int32_t j)
+{
+    if (j <= 0) return tmin;
+    if (j >= n) return tmax;
+    return tmin + (tmax - tmin) * inv_cdf8(cdf, (float)j / (float)n);
+}
+
+// t and w of sample j of a segment: uniform (HyperTree.cu:339-389) or inside the proposal's cell j.
+__device__ __forceinline__ void sample_tw(const ProjGeom& g, bool prop, const float (&cdf)[8], float tmin, float tmax,
+                                          float step, int32_t n, int32_t j, int64_t ray, int32_t rank, float& t,
+                                          float& w)
+{
+    const float a = seg_a(g.jitter, g.seed, ray, rank, j);
+    if (!prop)
+    {
+        t = sample_t(tmin, step, j, a);
+        w = sample_w(tmin, tmax, step, n, j, t, g.jitter, g.seed, ray, rank);
+        return;
+    }
+    const float t1 = proposal_cell(cdf, tmin, tmax, n, j);
+    const float t2 = proposal_cell(cdf, tmin, tmax, n, j + 1);
+    t              = t1 * (1.0f - a) + t2 * a;
+    w              = t2 - t1;
+}
+
+__device__ __forceinline__ bool load_cdf(const ProjGeom& g, int64_t k, float (&cdf)[8])
+{
+    if (!g.seg_cdf) return false;
+    const float4* r = reinterpret_cast<const float4*>(g.seg_cdf + 8 * k);
+    const float4 u = r[0], v = r[1];
+    cdf[0] = u.x; cdf[1] = u.y; cdf[2] = u.z; cdf[3] = u.w;
+    cdf[4] = v.x; cdf[5] = v.y; cdf[6] = v.z; cdf[7] = v.w;
+    return cdf[7] > 0.f;
+}
+
+__device__ __forceinline__ Sample sample_geo(const ProjGeom& g, int32_t k, int32_t j)
+{
+    Sample s;
+    const int64_t ray  = g.seg_ray[k];
+    const int32_t n    = g.seg_count[k];
+    const int32_t rank = (int32_t)(k - g.ray_seg_offset[ray]);
+    const int32_t node = g.seg_node[k];
+    const float tmin = g.seg_tmin[k], tmax = g.seg_tmax[k];
+    const float step = (tmax - tmin) / (float)n;
+    float cdf[8];
+    const bool prop = load_cdf(g, k, cdf);
+    float t;
+    sample_tw(g, prop, cdf, tmin, tmax, step, n, j, ray, rank, t, s.w);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        float p   = g.ro[3 * ray + d] + g.rd[3 * ray + d] * t;
+        float mn  = g.pmin[3 * node + d], mx = g.pmax[3 * node + d];
+        s.uv[d]   = (p - mn) / (mx - mn) * 2 - 1;
+    }
+    s.ray = ray;
+    return s;
+}
+
+// Trilinear corners in a cubic channels-last node grid (offsets in floats from the node's first voxel).
+struct Corners
+{
+    float w[8];
+    int32_t off[8];
+};
+
+__device__ __forceinline__ int clamp_g(int v, int G) { return min(G - 1, max(v, 0)); }
+
+// Continuous grid coordinate of a local coordinate (align_corners, IndirectGridSample3D.cu:448-455).
+__device__ __forceinline__ float grid_coord(float u, int G) { return ((u + 1) / 2) * (float)(G - 1); }
+
+__device__ __forceinline__ Corners corners_ix(float ix, float iy, float iz, int G, int C)
+{
+    Corners c;
+    int x0 = (int)floorf(ix), y0 = (int)floorf(iy), z0 = (int)floorf(iz);
+    int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+    c.w[0] = ((float)x1 - ix) * ((float)y1 - iy) * ((float)z1 - iz);
+    c.w[1] = (ix - (float)x0) * ((float)y1 - iy) * ((float)z1 - iz);
+    c.w[2] = ((float)x1 - ix) * (iy - (float)y0) * ((float)z1 - iz);
+    c.w[3] = (ix - (float)x0) * (iy - (float)y0) * ((float)z1 - iz);
+    c.w[4] = ((float)x1 - ix) * ((float)y1 - iy) * (iz - (float)z0);
+    c.w[5] = (ix - (float)x0) * ((float)y1 - iy) * (iz - (float)z0);
+    c.w[6] = ((float)x1 - ix) * (iy - (float)y0) * (iz - (float)z0);
+    c.w[7] = (ix - (float)x0) * (iy - (float)y0) * (iz - (float)z0);
+    const int cx0 = clamp_g(x0, G), cx1 = clamp_g(x1, G);
+    const int cy0 = clamp_g(y0, G), cy1 = clamp_g(y1, G);
+    const int cz0 = clamp_g(z0, G), cz1 = clamp_g(z1, G);
+    const int zs = G * G * C, ys = G * C;
+    c.off[0] = cz0 * zs + cy0 * ys + cx0 * C;
+    c.off[1] = cz0 * zs + cy0 * ys + cx1 * C;
+    c.off[2] = cz0 * zs + cy1 * ys + cx0 * C;
+    c.off[3] = cz0 * zs + cy1 * ys + cx1 * C;
+    c.off[4] = cz1 * zs + cy0 * ys + cx0 * C;
+    c.off[5] = cz1 * zs + cy0 * ys + cx1 * C;
+    c.off[6] = cz1 * zs + cy1 * ys + cx0 * C;
+    c.off[7] = cz1 * zs + cy1 * ys + cx1 * C;
+    return c;
+}
+
+__device__ __forceinline__ Corners corners_cl(const float uv[3], int G, int C)
+{
+    return corners_ix(grid_coord(uv[0], G), grid_coord(uv[1], G), grid_coord(uv[2], G), G, C);
+}
+
+template <int C>
+__device__ __forceinline__ void gather_cl(const float* node_grid, const Corners& c, float (&x)[C])
+{
+#pragma unroll
+    for (int q = 0; q < C; q += 4)
+    {
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            float4 v = *reinterpret_cast<const float4*>(node_grid + c.off[k] + q);
+            acc.x += v.x * c.w[k];
+            acc.y += v.y * c.w[k];
+            acc.z += v.z * c.w[k];
+            acc.w += v.w * c.w[k];
+        }
+        x[q]     = acc.x;
+        x[q + 1] = acc.y;
+        x[q + 2] = acc.z;
+        x[q + 3] = acc.w;
+    }
+}
+
+// Register-resident decoder C -> H -> H -> 1 (one hidden layer), packed parameters read with uniform loads.
+// Returns the output pre-activation.
+template <int C, int H, int ACT>
+__device__ __forceinline__ float mlp_forward_reg(const float* __restrict__ P, const float (&x)[C], float beta)
+{
+    const float* W0 = P;
+    const float* b0 = W0 + H * C;
+    const float* W1 = b0 + H;
+    const float* b1 = W1 + H * H;
+    const float* W2 = b1 + H;
+    const float* b2 = W2 + H;
+    float h0[H];
+#pragma unroll
+    for (int o = 0; o < H; ++o)
+    {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < C; ++k) acc = fmaf(W0[o * C + k], x[k], acc);
+        h0[o] = act_f(ACT, acc + b0[o], beta);
+    }
+    float acc2 = 0.f;
+#pragma unroll
+    for (int o = 0; o < H; ++o)
+    {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < H; ++k) acc = fmaf(W1[o * H + k], h0[k], acc);
+        float h1 = act_f(ACT, acc + b1[o], beta);
+        acc2     = fmaf(W2[o], h1, acc2);
+    }
+    return acc2 + b2[0];
+}
+
+// Node-major position of the segment holding chunk-relative sample s: the last q in [q0, q1) with
+// nm_soff[q] - base <= s.
+__device__ __forceinline__ int32_t find_segment(const int64_t* nm_soff, int32_t q0, int32_t q1, int64_t target)
+{
+    int32_t lo = q0, hi = q1 - 1;
+    while (lo < hi)
+    {
+        int32_t mid = (lo + hi + 1) >> 1;
+        if (nm_soff[mid] <= target)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---- matrix-core path ---------------------------------------------------------------------------------------------------
+// Gather of the lane's 4 channels (group g: channels 4g..4g+3) of an 8-channel node grid, FP32 or BF16 storage,
+// trilinear weights and accumulation in FP32 (same corner order as the reference).
+template <bool kBF16Grid>
+__device__ __forceinline__ void gather_half(const void* node_grid, const Corners& c, int g, float (&x4)[4])
+{
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+    {
+        float4 v;
+        if constexpr (kBF16Grid)
+        {
+            uint2 raw = *reinterpret_cast<const uint2*>(static_cast<const unsigned short*>(node_grid) + c.off[k] + 4 * g);
+            v = make_float4(__uint_as_float(raw.x << 16), __uint_as_float(raw.x & 0xFFFF0000u),
+                            __uint_as_float(raw.y << 16), __uint_as_float(raw.y & 0xFFFF0000u));
+        }
+        else
+        {
+            v = *reinterpret_cast<const float4*>(static_cast<const float*>(node_grid) + c.off[k] + 4 * g);
+        }
+        acc.x += v.x * c.w[k];
+        acc.y += v.y * c.w[k];
+        acc.z += v.z * c.w[k];
+        acc.w += v.w * c.w[k];
+    }
+    x4[0] = acc.x;
+    x4[1] = acc.y;
+    x4[2] = acc.z;
+    x4[3] = acc.w;
+}
+
+__device__ __forceinline__ const void* node_grid_ptr(const ProjGeom& g, const void* grid_lowp, bool bf16, int32_t slot)
+{
+    const int64_t off = (int64_t)slot * g.G * g.G * g.G * 8;
+    return bf16 ? static_cast<const void*>(static_cast<const unsigned short*>(grid_lowp) + off)
+                : static_cast<const void*>(g.grid + off);
+}
+
+// Segment state of one forward chunk, staged in LDS once per chunk: what the chunk's samples need from the segment
+// list and the rays, so that a sample's geometry costs LDS reads rather than chains of dependent global loads.
+struct ChunkCtx
+{
+    int32_t beg[kSegCap + 1];  // chunk-relative first sample of each segment; beg[nseg] = the chunk's sample count
+    int32_t k[kSegCap];        // canonical segment index
+    int32_t ray[kSegCap];
+    int32_t rank[kSegCap];
+    int32_t n[kSegCap];
+    float tmin[kSegCap], tmax[kSegCap], step[kSegCap];
+    float gproj[kSegCap];  // dL/dprojection of the segment's ray (backward)
+    float o[3][kSegCap], d[3][kSegCap];
+    float cdf[8][kSegCap];  // proposal CDF rows (cdf[7][t] == 0: uniform)
+    uint32_t hray[2][kSegCap];  // jitter_ray_hash of the segment's ray (low, high word)
+    uint16_t sq[kFwdChunk];  // window-relative sample -> chunk-relative segment
+};
+
+__device__ inline void load_chunk_ctx(const ProjGeom& g, const Plan& p, int32_t q0, int32_t q1, int64_t base,
+                               const float* grad_proj, ChunkCtx& c)
+{
+    const int32_t nseg = q1 - q0;
+    for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x)
+    {
+        const int32_t q   = q0 + t;
+        const int32_t k   = p.perm[q];
+        const int32_t ray = g.seg_ray[k];
+        const int32_t n   = g.seg_count[k];
+        const float tmin = g.seg_tmin[k], tmax = g.seg_tmax[k];
+        c.beg[t] = (int32_t)(p.nm_soff[q] - base);
+        if (t == nseg - 1) c.beg[nseg] = (int32_t)(p.nm_soff[q1] - base);
+        c.k[t]     = k;
+        c.ray[t]   = ray;
+        c.rank[t]  = (int32_t)(k - g.ray_seg_offset[ray]);
+        c.n[t]     = n;
+        c.tmin[t]  = tmin;
+        c.tmax[t]  = tmax;
+        c.step[t]  = (tmax - tmin) / (float)n;
+        c.gproj[t] = grad_proj ? grad_proj[ray] : 0.f;
+        const uint64_t hr = jitter_ray_hash(g.seed, ray);
+        c.hray[0][t]      = (uint32_t)hr;
+        c.hray[1][t]      = (uint32_t)(hr >> 32);
+        float cdf[8];
+        const bool prop = load_cdf(g, k, cdf);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) c.cdf[b][t] = prop ? cdf[b] : 0.f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            c.o[d][t] = g.ro[3 * (int64_t)ray + d];
+            c.d[d][t] = g.rd[3 * (int64_t)ray + d];
+        }
+    }
+}
+
+// sq for the window [w0, w0 + wn) of the chunk's samples (needs beg[] complete).
+__device__ inline void expand_window(ChunkCtx& c, int32_t nseg, int32_t w0, int32_t wn)
+{
+    for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x)
+    {
+        const int32_t a = max(c.beg[t], w0), b = min(c.beg[t + 1], w0 + wn);
+        for (int32_t i = a; i < b; ++i) c.sq[i - w0] = (uint16_t)t;
+    }
+}
+
+// Sample j of chunk segment t: the same operations as sample_geo on the same values, so the same bits.
+__device__ __forceinline__ Sample ctx_sample(const ProjGeom& g, const ChunkCtx& c, int32_t t, int32_t j,
+                                             const float (&mn)[3], const float (&mx)[3])
+{
+    Sample s;
+    const int64_t ray  = c.ray[t];
+    const int32_t rank = c.rank[t], n = c.n[t];
+    const float tmin = c.tmin[t], tmax = c.tmax[t], step = c.step[t];
+    const uint64_t hr = ((uint64_t)c.hray[1][t] << 32) | c.hray[0][t];
+    auto a_of         = [&](int32_t jj) { return g.jitter ? jitter_a_h(hr, rank, jj) : 0.5f; };
+    float tt;
+    if (c.cdf[7][t] > 0.f)
+    {
+        float cdf[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) cdf[b] = c.cdf[b][t];
+        const float t1 = proposal_cell(cdf, tmin, tmax, n, j);
+        const float t2 = proposal_cell(cdf, tmin, tmax, n, j + 1);
+        const float a  = a_of(j);
+        tt             = t1 * (1.0f - a) + t2 * a;
+        s.w            = t2 - t1;
+    }
+    else
+    {
+        // sample_t / sample_w with the ray's half of the hash taken from the context
+        tt             = sample_t(tmin, step, j, a_of(j));
+        const float lo = (j == 0) ? tmin : (sample_t(tmin, step, j - 1, a_of(j - 1)) + tt) * 0.5f;
+        const float hi = (j == n - 1) ? tmax : (sample_t(tmin, step, j + 1, a_of(j + 1)) + tt) * 0.5f;
+        s.w            = hi - lo;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        float pd = c.o[d][t] + c.d[d][t] * tt;
+        s.uv[d]  = (pd - mn[d]) / (mx[d] - mn[d]) * 2 - 1;
+    }
+    s.ray = ray;
+    return s;
+}
+
+__device__ __forceinline__ void node_bounds(const ProjGeom& g, int32_t node, float (&mn)[3], float (&mx)[3])
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        mn[d] = g.pmin[3 * node + d];
+        mx[d] = g.pmax[3 * node + d];
+    }
+}
+
+// Inputs of a pair of 32-sample tiles (samples p0 .. p0 + 63 of the window): every lane computes the geometry of one
+// sample (lane l: sample p0 + l) and the lane groups swap their corners, so that each lane holds the corners of
+// sample p0 + n (tile A) and p0 + 32 + n (tile B) and gathers its 4 channels of both. v is the sample weight
+// (forward) or dL/dy = dL/dproj x weight (backward); samples past the window have zero weights, features and v.
+struct PairIn
+{
+    float xa[4], xb[4];
+    float va, vb;
+};
+
+template <bool kBF16, bool kBwd>
+__device__ __forceinline__ void fetch_pair(const ProjGeom& g, const ChunkCtx& cx, int32_t p0, int32_t w0,
+                                           int32_t wn, int64_t base, const float (&mn)[3], const float (&mx)[3],
+                                           const void* ngrid, float4* sgc, PairIn& in)
+{
+    const int l = threadIdx.x & 63, hg = l >> 5;
+    const int32_t ii = p0 + l;
+    Corners cr;
+    float v = 0.f;
+    if (ii < wn)
+    {
+        const int32_t t = cx.sq[ii];
+        Sample smp      = ctx_sample(g, cx, t, w0 + ii - cx.beg[t], mn, mx);
+        const float ix = grid_coord(smp.uv[0], g.G), iy = grid_coord(smp.uv[1], g.G), iz = grid_coord(smp.uv[2], g.G);
+        cr = corners_ix(ix, iy, iz, g.G, 8);
+        v  = kBwd ? cx.gproj[t] * smp.w : smp.w;
+        if (kBwd) sgc[base + w0 + ii] = make_float4(ix, iy, iz, 0.f);
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            cr.off[k] = 0;
+            cr.w[k]   = 0.f;
+        }
+    }
+    Corners pc;  // the other lane group's sample
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+    {
+        pc.off[k] = __shfl_xor(cr.off[k], 32);
+        pc.w[k]   = __shfl_xor(cr.w[k], 32);
+    }
+    const float pv = __shfl_xor(v, 32);
+    Corners ca, cb;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+    {
+        ca.off[k] = hg == 0 ? cr.off[k] : pc.off[k];
+        ca.w[k]   = hg == 0 ? cr.w[k] : pc.w[k];
+        cb.off[k] = hg == 1 ? cr.off[k] : pc.off[k];
+        cb.w[k]   = hg == 1 ? cr.w[k] : pc.w[k];
+    }
+    gather_half<kBF16>(ngrid, ca, hg, in.xa);
+    gather_half<kBF16>(ngrid, cb, hg, in.xb);
+    in.va = hg == 0 ? v : pv;
+    in.vb = hg == 1 ? v : pv;
+}
+
+// ---- relu BF16 backward (projector_relu.hip) ----------------------------------------------------------------------
+// waves per SIMD the relu backward is compiled for (its register budget: 512 / waves per lane)
+#ifndef NEAT_RELU_BWD_WAVES
+#define NEAT_RELU_BWD_WAVES 2
+#endif
+
+// Launches k_mfma_bwd_relu<H> (H = 32 or 64) on nwg persistent workgroups of 256 threads.
+int launch_mfma_bwd_relu(hipStream_t s, int H, unsigned nwg, const ProjGeom& g, const Plan& p, const float* params,
+                         const void* grid_bf16, const float* grad_proj, float* dfeat, float4* sgc, unsigned int* dmax,
+                         float* partials, int32_t P_total);
+
+}  // namespace neat

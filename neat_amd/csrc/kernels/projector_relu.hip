@@ -1,0 +1,336 @@
+// Backward of the fused projector for BF16 decoders with relu hidden layers, in a file of its own: it is compiled
+// with the matrix-core results in VGPRs (-mllvm -amdgpu-mfma-vgpr-form), since nearly every product here is
+// consumed by vector ALU code, which cannot read accumulation registers directly.
+#include "projector_dev.hpp"
+
+namespace neat
+{
+
+// Backward for BF16 relu decoders (the configuration the projector benchmarks): as k_mfma_bwd, but every product is
+// formed on the matrix cores in the orientation its consumer needs (WeightLDSRelu), so no tile is transposed
+// through LDS. Per wave and tile of 32 samples, with "h" tiles holding hidden units in the registers and samples on
+// the lanes, and "s" tiles the transpose:
+//   forward  Z1h, Z1s = W0 [x;1]; H1 = relu(Z1) (BF16); Z2h, Z2s = W1 H1 + b1; z = w2 . relu(Z2h) + b2
+//   backward dz = dL/dz; D2 = dz * [Z2 > 0] (h and s; the output weights are folded into w2t and the final scale)
+//            dW1 += D2s H1s^T; E += D2s [e10] + relu(Z2s) [dz e9] + D1s [x; 1]  (bias / output / first-layer grads)
+//            D1h, D1s = (w2t D2h) * [H1 > 0]; dL/dfeature = W0^T D1h
+// The relu masks come from the BF16 activations themselves, so only BF16 fragments stay live across the tile.
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+// Elementwise work on BF16 fragments runs as packed 16-bit integer operations on their bit patterns (two lanes of
+// a register per instruction; integer ops, not packed FP32): a BF16 value is negative exactly when its int16
+// pattern is, and after relu a nonzero pattern means an active unit.
+
+// BF16 fragment of z slots 8st..8st+7, rounded to nearest even, one conversion per pair (converting a 2-vector;
+// converting single values makes the compiler convert each alone and repack the halves)
+__device__ __forceinline__ bf16x8 bf16_frag(const f32x16& z, int st)
+{
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 r;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+        const f32x2 v = {z[8 * st + 2 * q], z[8 * st + 2 * q + 1]};
+        r[q]          = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+    }
+    return __builtin_bit_cast(bf16x8, r);
+}
+
+// relu on BF16 patterns: max(int16, 0) maps every negative value, -0 included, to +0
+__device__ __forceinline__ bf16x8 relu_bf16(const bf16x8& f)
+{
+    return __builtin_bit_cast(bf16x8, __builtin_elementwise_max(__builtin_bit_cast(s16x8, f), (s16x8)0));
+}
+
+// f where the relu activation h is nonzero, else 0: f * min(h, 1) per 16-bit lane, two lanes per instruction. The
+// constant 1 is hidden from the compiler (an SGPR set by an s_mov it cannot see through); knowing that min(h, 1) is
+// 0 or 1 it would rewrite the product as per-lane compares and selects.
+__device__ __forceinline__ uint32_t opaque_one2()
+{
+    uint32_t v;
+    asm volatile("s_mov_b32 %0, 0x10001" : "=s"(v));
+    return v;
+}
+
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8 keep_active(const bf16x8& f, const bf16x8& h, uint32_t one2)
+{
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 ones = {one2, one2, one2, one2};
+    const u16x8 m    = __builtin_elementwise_min(__builtin_bit_cast(u16x8, h), __builtin_bit_cast(u16x8, ones));
+    return __builtin_bit_cast(bf16x8, __builtin_bit_cast(u16x8, f) * m);
+}
+
+// relu for the FP32 output-layer dot product (one instruction; -0 may pass through, which adds nothing)
+__device__ __forceinline__ float relu_f32(float z) { return __builtin_amdgcn_fmed3f(z, 0.f, 3.402823466e38f); }
+
+template <int H>
+__global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(ProjGeom g, Plan p, const float* __restrict__ params,
+                                                          const void* grid_lowp, const float* grad_proj, float* dfeat,
+                                                          float4* sgc, unsigned int* dmax, float* partials,
+                                                          int32_t P_total)
+{
+    constexpr int MT = H / 32, C = 8;
+    typedef PrecBF16 P;
+    __shared__ WeightLDSRelu<H> w;
+    __shared__ ChunkCtx cx;
+    __shared__ __align__(16) float s_dz[4][32];
+    load_weights_relu<H>(params, w);
+    __syncthreads();  // the final partial write reads w.w3 even in a workgroup that gets no chunk
+    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
+    // constant fragments: ONES (group 0 slots 0, 1) against the hidden bias; IDB routes the slots of the feature
+    // fragment to columns of the s-tile T (channels 0..7, 1 -> 8, 1 -> 10, dz -> 9)
+    bf16x8 ones = {}, idb = {};
+    if (hg == 0)
+    {
+        ones[0] = (__bf16)1.f;
+        ones[1] = (__bf16)1.f;
+    }
+    {
+        const int col[2][8] = {{0, 1, 2, 3, 8, 10, 9, -1}, {4, 5, 6, 7, -1, -1, -1, -1}};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) idb[j] = (__bf16)(col[hg][j] == n ? 1.f : 0.f);
+    }
+    f32x16 dW2[MT][MT], E[MT];
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+    {
+        E[a] = f32x16{};
+#pragma unroll
+        for (int b = 0; b < MT; ++b) dW2[a][b] = f32x16{};
+    }
+    float b3 = 0.f;
+    float amax = 0.f;
+    float* sdz          = s_dz[wave];
+    const uint32_t one2 = opaque_one2();
+    const int64_t nchunks = p.chunk_off1[g.N];
+    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x)
+    {
+        const int32_t slot = p.chunk1_slot[c];
+        const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c], nseg = q1 - q0;
+        if (nseg == 0) continue;
+        const int64_t base = p.nm_soff[q0];
+        __syncthreads();
+        load_chunk_ctx(g, p, q0, q1, base, grad_proj, cx);
+        float mn[3], mx[3];
+        node_bounds(g, g.seg_node[p.perm[q0]], mn, mx);
+        const void* ngrid = node_grid_ptr(g, grid_lowp, true, slot);
+        __syncthreads();
+        const int32_t nsamp = cx.beg[nseg];
+        for (int32_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
+        {
+            const int32_t wn = min(kFwdChunk, nsamp - w0);
+            expand_window(cx, nseg, w0, wn);
+            __syncthreads();
+            auto tile = [&](int32_t i, const float (&x4)[4], float gy) {
+                const bool valid = i < wn;
+                // feature fragment: channels 4hg..4hg+3, group 0 adds the two bias slots (1, 1)
+                bf16x8 X = {};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) X[j] = (__bf16)x4[j];
+                if (hg == 0)
+                {
+                    X[4] = (__bf16)1.f;
+                    X[5] = (__bf16)1.f;
+                }
+                // first layer, both orientations
+                bf16x8 H1h[MT][2], H1s[MT][2];
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+                    const f32x16 zh = P::mma(w.w1e[mt][l], X, f32x16{});
+                    const f32x16 zs = P::mma(X, w.w1e[mt][l], f32x16{});
+#pragma unroll
+                    for (int st = 0; st < 2; ++st)
+                    {
+                        H1h[mt][st] = relu_bf16(bf16_frag(zh, st));
+                        H1s[mt][st] = relu_bf16(bf16_frag(zs, st));
+                    }
+                }
+                // hidden layer (h): output pre-activation and dz
+                bf16x8 D2h[MT][2];
+                float dz;
+                {
+                    f32x16 Z2h[MT];
+                    float part = 0.f;
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                    {
+                        f32x16 acc = P::mma(w.w2b[mt][l], ones, f32x16{});
+#pragma unroll
+                        for (int ms = 0; ms < MT; ++ms)
+                        {
+#pragma unroll
+                            for (int st = 0; st < 2; ++st) acc = P::mma(w.w2[mt][ms][st][l], H1h[ms][st], acc);
+                        }
+                        Z2h[mt] = acc;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                        {
+                            const float4 w3 = *reinterpret_cast<const float4*>(&w.w3[32 * mt + 8 * q + 4 * hg]);
+                            part = fmaf(w3.x, relu_f32(acc[4 * q + 0]), part);
+                            part = fmaf(w3.y, relu_f32(acc[4 * q + 1]), part);
+                            part = fmaf(w3.z, relu_f32(acc[4 * q + 2]), part);
+                            part = fmaf(w3.w, relu_f32(acc[4 * q + 3]), part);
+                        }
+                    }
+                    const float z = part + __shfl_xor(part, 32) + w.b3;
+                    dz = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, z, g.beta) : 0.f;
+                    bf16x8 dz8;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) dz8[j] = (__bf16)dz;
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                    {
+#pragma unroll
+                        for (int st = 0; st < 2; ++st) D2h[mt][st] = keep_active(dz8, relu_bf16(bf16_frag(Z2h[mt], st)), one2);
+                    }
+                }
+                if (hg == 0) b3 += dz;
+                // dz per register of an s tile (sample crow(i, hg)), through LDS
+                wave_lds_fence();
+                if (hg == 0) sdz[n] = dz;
+                wave_lds_fence();
+                bf16x8 dzs[2];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    const float4 v = *reinterpret_cast<const float4*>(&sdz[8 * q + 4 * hg]);
+                    dzs[q >> 1][4 * (q & 1) + 0] = (__bf16)v.x;
+                    dzs[q >> 1][4 * (q & 1) + 1] = (__bf16)v.y;
+                    dzs[q >> 1][4 * (q & 1) + 2] = (__bf16)v.z;
+                    dzs[q >> 1][4 * (q & 1) + 3] = (__bf16)v.w;
+                }
+                // the s tile T of [x; 1; dz; 1] (columns 0..7, 8, 9, 10) and its masked B fragments
+                bf16x8 B08[2], B9[2], B10[2];
+                {
+                    if (hg == 0) X[6] = (__bf16)dz;
+                    const f32x16 T = P::mma(X, idb, f32x16{});
+#pragma unroll
+                    for (int st = 0; st < 2; ++st)
+                    {
+                        bf16x8 t;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) t[j] = (__bf16)T[8 * st + j];
+                        B08[st] = n <= 8 ? t : bf16x8{};
+                        B9[st]  = n == 9 ? t : bf16x8{};
+                        B10[st] = n == 10 ? t : bf16x8{};
+                    }
+                }
+                // hidden layer (s): relu(Z2s) and D2s; output-weight and hidden-bias gradients; hidden-weight gradient
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+                    f32x16 acc = P::mma(ones, w.w2b[mt][l], f32x16{});
+#pragma unroll
+                    for (int ms = 0; ms < MT; ++ms)
+                    {
+#pragma unroll
+                        for (int st = 0; st < 2; ++st) acc = P::mma(H1h[ms][st], w.w2[mt][ms][st][l], acc);
+                    }
+#pragma unroll
+                    for (int st = 0; st < 2; ++st)
+                    {
+                        const bf16x8 h2 = relu_bf16(bf16_frag(acc, st)), d2 = keep_active(dzs[st], h2, one2);
+                        E[mt] = P::mma(h2, B9[st], E[mt]);
+                        E[mt] = P::mma(d2, B10[st], E[mt]);
+#pragma unroll
+                        for (int m1 = 0; m1 < MT; ++m1) dW2[mt][m1] = P::mma(d2, H1s[m1][st], dW2[mt][m1]);
+                    }
+                }
+                // first layer (s): D1s and the first-layer weight / bias gradients
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+                    f32x16 acc = {};
+#pragma unroll
+                    for (int ms = 0; ms < MT; ++ms)
+                    {
+#pragma unroll
+                        for (int st = 0; st < 2; ++st) acc = P::mma(D2h[ms][st], w.w2t[mt][ms][st][l], acc);
+                    }
+#pragma unroll
+                    for (int st = 0; st < 2; ++st) E[mt] = P::mma(keep_active(bf16_frag(acc, st), H1s[mt][st], one2), B08[st], E[mt]);
+                }
+                // first layer (h): D1h and dL/dfeature = W0^T D1h (rows 0..7: registers 0..3 of each lane group)
+                f32x16 dx = {};
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+                    f32x16 acc = {};
+#pragma unroll
+                    for (int ms = 0; ms < MT; ++ms)
+                    {
+#pragma unroll
+                        for (int st = 0; st < 2; ++st) acc = P::mma(w.w2t[mt][ms][st][l], D2h[ms][st], acc);
+                    }
+#pragma unroll
+                    for (int st = 0; st < 2; ++st) dx = P::mma(w.w1t[mt][st][l], keep_active(bf16_frag(acc, st), H1h[mt][st], one2), dx);
+                }
+                if (valid)
+                {
+                    *reinterpret_cast<float4*>(dfeat + ((int64_t)hg * g.S + base + w0 + i) * 4) =
+                        make_float4(dx[0], dx[1], dx[2], dx[3]);
+                    amax = fmaxf(amax, fmaxf(fmaxf(fabsf(dx[0]), fabsf(dx[1])), fmaxf(fabsf(dx[2]), fabsf(dx[3]))));
+                }
+            };
+            for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
+            {
+                PairIn cur;
+                fetch_pair<true, true>(g, cx, p0, w0, wn, base, mn, mx, ngrid, sgc, cur);
+#pragma unroll 1
+                for (int h = 0; h < 2; ++h)
+                {
+                    if (h == 1 && p0 + 32 >= wn) break;
+                    float xs[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) xs[j] = h ? cur.xb[j] : cur.xa[j];
+                    tile(p0 + 32 * h + n, xs, h ? cur.vb : cur.va);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // ---- this wave's slice of the partial parameter gradient (packed layout); rows scaled by the output weights
+    // where the output weights were folded out of D2 ----
+    float* part   = partials + ((int64_t)blockIdx.x * 4 + wave) * P_total;
+    const int oW0 = 0, ob0 = H * C, oW1 = ob0 + H, ob1 = oW1 + H * H, oW2 = ob1 + H, ob2 = oW2 + H;
+#pragma unroll
+    for (int m2 = 0; m2 < MT; ++m2)
+    {
+#pragma unroll
+        for (int i2 = 0; i2 < 16; ++i2)
+        {
+            const int r    = 32 * m2 + crow(i2, hg);
+            const float s3 = w.w3[r];
+#pragma unroll
+            for (int m1 = 0; m1 < MT; ++m1) part[oW1 + r * H + 32 * m1 + n] = dW2[m2][m1][i2] * s3;
+            if (n < C) part[oW0 + r * C + n] = E[m2][i2];
+            if (n == 8) part[ob0 + r] = E[m2][i2];
+            if (n == 9) part[oW2 + r] = E[m2][i2];
+            if (n == 10) part[ob1 + r] = E[m2][i2] * s3;
+        }
+    }
+#pragma unroll
+    for (int off = 16; off >= 1; off >>= 1) b3 += __shfl_xor(b3, off);
+    if (l == 0) part[ob2] = b3;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off));
+    if (l == 0) atomicMax(dmax, __float_as_uint(amax));
+}
+
+int launch_mfma_bwd_relu(hipStream_t s, int H, unsigned nwg, const ProjGeom& g, const Plan& p, const float* params,
+                         const void* grid_bf16, const float* grad_proj, float* dfeat, float4* sgc, unsigned int* dmax,
+                         float* partials, int32_t P_total)
+{
+    if (H == 64)
+        k_mfma_bwd_relu<64><<<nwg, 256, 0, s>>>(g, p, params, grid_bf16, grad_proj, dfeat, sgc, dmax, partials, P_total);
+    else
+        k_mfma_bwd_relu<32><<<nwg, 256, 0, s>>>(g, p, params, grid_bf16, grad_proj, dfeat, sgc, dmax, partials, P_total);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace neat

@@ -459,7 +459,7 @@ extern "C"
         float* part    = static_cast<float*>(workspace);
         k_prop_bwd<<<nwg, 256, 0, hs>>>(s, params, target, scale, part);
         NEAT_LAUNCHED();
-        k_dec_reduce_partials<<<grid_for(kPParams, 256), 256, 0, hs>>>(part, nwg * 4, kPParams, grad_params);
+        k_dec_reduce_partials<<<reduce_partials_blocks(kPParams), 256, 0, hs>>>(part, nwg * 4, kPParams, grad_params);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }

@@ -37,10 +37,13 @@ __device__ __forceinline__ int32_t segment_count(const float mn[3], const float 
     return (int32_t)ceilf(rel * (float)max_spn);
 }
 
+// Children are visited in the order i ^ mask (bit k of mask set when the ray runs towards -axis k), a front-to-back
+// order for most rays, so segments mostly arrive sorted by entry t; the order does not change what is visited.
 template <typename Visit>
 __device__ __forceinline__ void traverse(const TreeArgs& T, const float o[3], const float d[3], Visit&& visit)
 {
-    int32_t n = 0;
+    const int32_t mask = (d[0] < 0.f ? 1 : 0) | (d[1] < 0.f ? 2 : 0) | (d[2] < 0.f ? 4 : 0);
+    int32_t n          = 0;
     while (true)
     {
         uint8_t f    = T.flags[n];
@@ -59,12 +62,13 @@ __device__ __forceinline__ void traverse(const TreeArgs& T, const float o[3], co
         }
         if (descend)
         {
-            n = 8 * n + 1;
+            n = 8 * n + 1 + mask;
             continue;
         }
-        while (n != 0 && ((n - 1) & 7) == 7) n = (n - 1) >> 3;
+        // up while n is the last child in visit order, then to the next sibling
+        while (n != 0 && (((n - 1) & 7) ^ mask) == 7) n = (n - 1) >> 3;
         if (n == 0) break;
-        ++n;
+        n = ((n - 1) & ~7) + 1 + (((((n - 1) & 7) ^ mask) + 1) ^ mask);
     }
 }
 
@@ -120,14 +124,30 @@ __global__ void __launch_bounds__(128) k_ray_segments(TreeArgs T, const float* r
     const int64_t base = ray_seg_offset[r];
     const int64_t end  = ray_seg_offset[r + 1];
     int64_t k          = base;
-    // insertion into the sorted prefix [base, k) while walking (segments per ray are few)
+    float last_t       = 0.f;  // the largest (tmin, node) so far, i.e. the entry at k - 1 (valid when k > base)
+    int32_t last_n     = 0;
+    // insertion into the sorted prefix [base, k) while walking (segments per ray are few, and mostly arrive in
+    // order, which the register copy of the last entry answers without a memory read)
     traverse(T, o, d,
              [&](int32_t node, float tmin, float tmax, const float* mn, const float* mx)
              {
                  if (k >= end) return;  // cannot happen when the offsets come from k_ray_count
                  int32_t cnt = segment_count(mn, mx, tmin, tmax, max_spn);
                  int64_t p   = k;
-                 while (p > base && seg_less(tmin, node, seg_tmin[p - 1], seg_node[p - 1]))
+                 if (k == base || !seg_less(tmin, node, last_t, last_n))
+                 {
+                     last_t = tmin;
+                     last_n = node;
+                 }
+                 else
+                 {
+                     seg_tmin[p]  = last_t;
+                     seg_tmax[p]  = seg_tmax[p - 1];
+                     seg_node[p]  = last_n;
+                     seg_count[p] = seg_count[p - 1];
+                     --p;
+                 }
+                 while (p > base && p < k && seg_less(tmin, node, seg_tmin[p - 1], seg_node[p - 1]))
                  {
                      seg_tmin[p]  = seg_tmin[p - 1];
                      seg_tmax[p]  = seg_tmax[p - 1];
