@@ -13,6 +13,8 @@
 // C: crow(i, l/32), column l%32).
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace neat
@@ -296,17 +298,24 @@ __device__ __forceinline__ typename P::Frag stage_frag(const float* buf, int row
 //                        dst = first-layer unit 32mt + l%32, src = hidden unit 32ms + crow(8st + j, l/32)
 //   w1t[MT][2][64]       transposed first layer (as WeightLDS::w1t)
 //   w3[64], b3           output layer (FP32)
+// WeightLDSReluFwd holds what the forward reads (w1e, w2, w2b, w3, b3); WeightLDSRelu adds w2t and w1t.
 template <int H>
-struct WeightLDSRelu
+struct WeightLDSReluFwd  // what the forward needs
 {
     static constexpr int MT = H / 32;
     bf16x8 w1e[MT][64];
     bf16x8 w2[MT][MT][2][64];
     bf16x8 w2b[MT][64];
-    bf16x8 w2t[MT][MT][2][64];
-    bf16x8 w1t[MT][2][64];
     float w3[64];
     float b3;
+};
+
+template <int H>
+struct WeightLDSRelu : WeightLDSReluFwd<H>
+{
+    static constexpr int MT = H / 32;
+    bf16x8 w2t[MT][MT][2][64];
+    bf16x8 w1t[MT][2][64];
 };
 
 // (hi, lo) BF16 split of an FP32 value into slots k, k + 1 of a fragment: hi + lo carries about 16 significant bits
@@ -318,9 +327,10 @@ __device__ __forceinline__ void bf16_split(float v, bf16x8& f, int k)
 }
 
 // Packed parameter layout as load_weights (C = 8).
-template <int H>
-__device__ void load_weights_relu(const float* __restrict__ params, WeightLDSRelu<H>& w)
+template <int H, typename WS>
+__device__ void load_weights_relu(const float* __restrict__ params, WS& w)
 {
+    constexpr bool kBwd = std::is_same<WS, WeightLDSRelu<H>>::value;
     constexpr int MT = H / 32, C = 8;
     const float* W0 = params;
     const float* b0 = W0 + H * C;
@@ -354,9 +364,10 @@ __device__ void load_weights_relu(const float* __restrict__ params, WeightLDSRel
             a[j]          = (__bf16)W1[dst * H + src];
             b[j]          = (__bf16)(W1[src * H + dst] * W2[src]);
         }
-        w.w2[mt][ms][st][l]  = a;
-        w.w2t[mt][ms][st][l] = b;
+        w.w2[mt][ms][st][l] = a;
+        if constexpr (kBwd) w.w2t[mt][ms][st][l] = b;
     }
+    if constexpr (kBwd)
     for (int t = threadIdx.x; t < MT * 2 * 64; t += blockDim.x)
     {
         const int l = t % 64, st = (t / 64) % 2, ms = t / 128;

@@ -1056,6 +1056,12 @@ template <typename P, int H>
 int launch_fwd_mfma(hipStream_t s, const Args& a, const Plan& p, float* seg_partial, float* sample_out = nullptr)
 {
     const unsigned grid = (unsigned)mfma_fwd_workgroups(a.z);
+    if (P::kBF16 && a.act == NEAT_ACT_RELU)
+    {
+        NEAT_REQUIRE(launch_mfma_fwd_relu(s, H, grid, a.g, p, a.params, a.grid_bf16, seg_partial, sample_out) == 0,
+                     "k_mfma_fwd_relu launch failed");
+        return NEAT_OK;
+    }
     if (a.act == NEAT_ACT_RELU)
         k_mfma_fwd<P, H, NEAT_ACT_RELU><<<grid, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, seg_partial, sample_out);
     else

@@ -484,6 +484,10 @@ __device__ __forceinline__ void fetch_pair(const ProjGeom& g, const ChunkCtx& cx
 #define NEAT_RELU_BWD_WAVES 2
 #endif
 
+// Launches k_mfma_fwd_relu<H> (H = 32 or 64) on nwg persistent workgroups of 256 threads.
+int launch_mfma_fwd_relu(hipStream_t s, int H, unsigned nwg, const ProjGeom& g, const Plan& p, const float* params,
+                         const void* grid_bf16, float* seg_partial, float* sample_out);
+
 // Launches k_mfma_bwd_relu<H> (H = 32 or 64) on nwg persistent workgroups of 256 threads.
 int launch_mfma_bwd_relu(hipStream_t s, int H, unsigned nwg, const ProjGeom& g, const Plan& p, const float* params,
                          const void* grid_bf16, const float* grad_proj, float* dfeat, float4* sgc, unsigned int* dmax,

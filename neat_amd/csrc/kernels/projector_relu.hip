@@ -322,6 +322,125 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
     if (l == 0) atomicMax(dmax, __float_as_uint(amax));
 }
 
+// Forward for BF16 relu decoders: k_mfma_fwd's structure (persistent workgroups over the chunks, tiles of 32
+// samples per wave, per-segment sums sequential in j) with the first- and hidden-layer biases in the products'
+// spare k slots and the elementwise work on packed BF16 fragments.
+template <int H>
+__global__ void __launch_bounds__(256, 2) k_mfma_fwd_relu(ProjGeom g, Plan p, const float* __restrict__ params,
+                                                          const void* grid_lowp, float* seg_partial, float* sample_out)
+{
+    constexpr int MT = H / 32;
+    typedef PrecBF16 P;
+    __shared__ WeightLDSReluFwd<H> w;
+    __shared__ ChunkCtx cx;
+    __shared__ float s_contrib[kFwdChunk];
+    __shared__ float s_seg[kSegCap];
+    load_weights_relu<H>(params, w);
+    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
+    bf16x8 ones = {};
+    if (hg == 0)
+    {
+        ones[0] = (__bf16)1.f;
+        ones[1] = (__bf16)1.f;
+    }
+    const int64_t nchunks = p.chunk_off1[g.N];
+    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x)
+    {
+        const int32_t slot = p.chunk1_slot[c];
+        const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c], nseg = q1 - q0;
+        if (nseg == 0) continue;
+        const int64_t base = p.nm_soff[q0];
+        __syncthreads();  // the previous chunk's readers of cx / s_seg are done (and the weights are in place)
+        load_chunk_ctx(g, p, q0, q1, base, nullptr, cx);
+        for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x) s_seg[t] = 0.f;
+        float mn[3], mx[3];
+        node_bounds(g, g.seg_node[p.perm[q0]], mn, mx);
+        const void* ngrid = node_grid_ptr(g, grid_lowp, true, slot);
+        __syncthreads();
+        const int32_t nsamp = cx.beg[nseg];
+        for (int32_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
+        {
+            const int32_t wn = min(kFwdChunk, nsamp - w0);
+            expand_window(cx, nseg, w0, wn);
+            __syncthreads();
+            auto tile = [&](int32_t i, const float (&x4)[4], float wgt) {
+                bf16x8 X = {};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) X[j] = (__bf16)x4[j];
+                if (hg == 0)
+                {
+                    X[4] = (__bf16)1.f;
+                    X[5] = (__bf16)1.f;
+                }
+                bf16x8 H1h[MT][2];
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+                    const f32x16 zh = P::mma(w.w1e[mt][l], X, f32x16{});
+#pragma unroll
+                    for (int st = 0; st < 2; ++st) H1h[mt][st] = relu_bf16(bf16_frag(zh, st));
+                }
+                float part = 0.f;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                {
+                    f32x16 acc = P::mma(w.w2b[mt][l], ones, f32x16{});
+#pragma unroll
+                    for (int ms = 0; ms < MT; ++ms)
+                    {
+#pragma unroll
+                        for (int st = 0; st < 2; ++st) acc = P::mma(w.w2[mt][ms][st][l], H1h[ms][st], acc);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                    {
+                        const float4 w3 = *reinterpret_cast<const float4*>(&w.w3[32 * mt + 8 * q + 4 * hg]);
+                        part = fmaf(w3.x, relu_f32(acc[4 * q + 0]), part);
+                        part = fmaf(w3.y, relu_f32(acc[4 * q + 1]), part);
+                        part = fmaf(w3.z, relu_f32(acc[4 * q + 2]), part);
+                        part = fmaf(w3.w, relu_f32(acc[4 * q + 3]), part);
+                    }
+                }
+                const float z = part + __shfl_xor(part, 32) + w.b3;
+                if (i < wn && hg == 0)
+                {
+                    const float y = act_f(NEAT_ACT_SOFTPLUS, z, g.beta);
+                    s_contrib[i]  = y * wgt;
+                    if (sample_out) sample_out[base + w0 + i] = y;
+                }
+            };
+            for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
+            {
+                PairIn cur;
+                fetch_pair<true, false>(g, cx, p0, w0, wn, base, mn, mx, ngrid, nullptr, cur);
+                tile(p0 + n, cur.xa, cur.va);
+                if (p0 + 32 < wn) tile(p0 + 32 + n, cur.xb, cur.vb);
+            }
+            __syncthreads();
+            for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x)
+            {
+                const int32_t a = max(cx.beg[t], w0), b = min(cx.beg[t + 1], w0 + wn);
+                if (a >= b) continue;
+                float acc = 0.f;
+                for (int32_t i = a; i < b; ++i) acc += s_contrib[i - w0];
+                s_seg[t] += acc;
+            }
+            __syncthreads();
+        }
+        for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x) seg_partial[cx.k[t]] = s_seg[t];
+    }
+}
+
+int launch_mfma_fwd_relu(hipStream_t s, int H, unsigned nwg, const ProjGeom& g, const Plan& p, const float* params,
+                         const void* grid_bf16, float* seg_partial, float* sample_out)
+{
+    if (H == 64)
+        k_mfma_fwd_relu<64><<<nwg, 256, 0, s>>>(g, p, params, grid_bf16, seg_partial, sample_out);
+    else
+        k_mfma_fwd_relu<32><<<nwg, 256, 0, s>>>(g, p, params, grid_bf16, seg_partial, sample_out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_mfma_bwd_relu(hipStream_t s, int H, unsigned nwg, const ProjGeom& g, const Plan& p, const float* params,
                          const void* grid_bf16, const float* grad_proj, float* dfeat, float4* sgc, unsigned int* dmax,
                          float* partials, int32_t P_total)
