@@ -37,11 +37,14 @@ def parse():
     ap.add_argument("--cpu-rays", type=int, default=12288, help="rays in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--proposal", action="store_true",
-                    help="BF16 + learned proposal form: the field is the phantom fitted on the leaf grids, segments "
-                         "with more than 8 uniform samples get 8 proposal-guided ones, and the proposal network takes "
-                         "one online training step every --proposal-train-every steps on 1/32 of the batch's rays; reports the projection error of the "
-                         "form against the FP64 oracle with uniform samples")
+    ap.add_argument("--sampling", default="auto", choices=["auto", "proposal", "uniform"],
+                    help="proposal (the default for the fused BF16 path): BF16 + learned proposal form; the field is "
+                         "the phantom fitted on the leaf grids, segments with more than 8 uniform samples get 8 "
+                         "proposal-guided ones unless their slot's observed variation exceeds --proposal-tau, and the "
+                         "proposal network takes one online training step every --proposal-train-every steps on 1/32 "
+                         "of the batch's rays; the projection error is reported against the FP64 oracle with uniform "
+                         "samples. uniform (the default otherwise): the reference's uniform in-node sampling over a "
+                         "random N(0, 0.1) grid.")
     ap.add_argument("--proposal-tau", type=float, default=0.3,
                     help="fallback threshold: segments in slots whose observed field variation exceeds it keep their "
                          "uniform samples")
@@ -69,6 +72,9 @@ def copy_calibration(mib):
 
 def main():
     a = parse()
+    if a.sampling == "auto":
+        a.sampling = "proposal" if (a.path == "fused" and a.precision == "bf16") else "uniform"
+    a.proposal = a.sampling == "proposal"
     if a.copy_calibration:
         return copy_calibration(a.copy_calibration)
     if a.gpus != 1:
@@ -82,8 +88,8 @@ def main():
     dev = "cuda"
     scene = W.make_scene(cfg, seed=a.seed)
     if a.proposal:
-        if a.path != "fused":
-            raise SystemExit("--proposal needs the fused path")
+        if a.path != "fused" or a.precision != "bf16":
+            raise SystemExit("--sampling proposal needs the fused BF16 path")
         scene["grid"], scene["params"] = W.phantom_field(scene, cfg)
     ds = pipeline.to_device(scene, cfg, dev)
     batches = []
@@ -239,7 +245,7 @@ def main():
     achieved = amount / secs / (1e12 if unit == "TFLOP/s" else 1e9)
     roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
                 "frac": round(achieved / peak, 5),
-                "traffic": None if prop is not None else measured_traffic(kname), "kernel": kname,
+                "traffic": measured_traffic(workload_name(a), kname), "kernel": kname,
                 "stage": dominant, "kernel_ms": round(stage_ms[dominant], 4)}
 
     cpu = None
@@ -247,7 +253,7 @@ def main():
         cpu = cpu_baseline(scene, cfg, a.cpu_rays, a.seed)
     err = projection_error(scene, cfg, ds, prop, a, grid_bf16)
 
-    prec_name = (a.precision + ("+proposal" if prop is not None else "")) if a.path == "fused" else "fp32_unfused"
+    prec_name = workload_name(a)
     out = {
         "metric": "fwd+bwd rays/s at 1 GPU, BF16+proposal vs plain FP32, at <=1e-3 rel err vs FP64",
         "value": round(rays_per_s, 1),
@@ -300,16 +306,24 @@ def projection_error(scene, cfg, ds, prop, a, grid_bf16, nrays=2048):
             "judge": "FP64 oracle, uniform samples (max 64 per node), same segments"}
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary (FETCH_SIZE, WRITE_SIZE in KiB
-    per dispatch, each scaled by the reading of a plain copy of known size), or None when not profiled."""
+def workload_name(a):
+    """Name of the measured form: bf16+proposal, bf16, fp32, fp32_valu or fp32_unfused."""
+    if a.path != "fused":
+        return "fp32_unfused"
+    return a.precision + ("+proposal" if a.proposal else "")
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` in `workload` from the committed rocprofv3 counter summary (FETCH_SIZE,
+    WRITE_SIZE in KiB per dispatch, each scaled by the reading of a plain copy of known size), or None when that
+    workload was not profiled."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    k = t.get("kernels", {}).get(kernel)
+    k = t.get("workloads", {}).get(workload, {}).get("kernels", {}).get(kernel)
     if not k:
         return None
     return round(1024 * (k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"]))
