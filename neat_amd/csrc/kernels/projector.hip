@@ -333,7 +333,7 @@ constexpr int kScatterThreads = 1024;
 // of one instruction lie far apart along the node-major order (different segments / cells: few same-address
 // collisions); a stream walks its samples in order and reloads the segment state only at segment boundaries.
 // 2^F with F = 45 - exponent(max |dL/dfeature|): every contribution is below 2^45 in fixed point and a vertex's sum
-// of fewer than 2^18 of them stays inside int64; quantisation is 2^-46 of the largest value per contribution.
+// of fewer than 2^18 of them stays inside int64; quantisation is 2^-46 of the largest value per converted sum.
 __device__ __forceinline__ double fixed_scale(unsigned int maxbits)
 {
     const float m = __uint_as_float(maxbits);
@@ -376,8 +376,11 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
         // order-independent, so the result is bit-reproducible, and LDS integer atomics run several times faster
         // than float ones. Lanes are (sample stream, corner) pairs: the 8 lanes of a stream share its sample's loads,
         // each computes one corner's weight and vertex once and adds all 4 channels of the half. The chunk's samples
-        // are cut into 8 x 16 contiguous streams.
+        // are cut into 8 x 16 contiguous streams. The LDS sums are planar, [channel][vertex]: one atomic instruction
+        // adds one channel for 64 vertices, which then spread over all bank pairs (interleaved channels would put
+        // every lane of an instruction on the same eighth of the banks).
         unsigned long long* sq = reinterpret_cast<unsigned long long*>(sg);
+        const int32_t nv       = g.G * g.G * g.G;
         const int sub = lane >> 3, kk = lane & 7;
         const int64_t U2 = (int64_t)(blockDim.x >> 6) * 8;
         const int64_t L2 = (ns + U2 - 1) / U2;
@@ -387,17 +390,20 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
         const float4* dh   = reinterpret_cast<const float4*>(dfeat) + (int64_t)blockIdx.y * g.S + base;
         if (s2 < e2)
         {
-            // Consecutive samples of a stream usually fall in the same cell: a lane keeps its corner's 4 sums in
-            // registers while the vertex stays the same and adds them to LDS when it changes (exact: integers either
-            // way). Loads go in blocks of kSB samples, the next block in flight while the current one is summed.
+            // Consecutive samples of a stream usually fall in the same cell (a run of about 4 samples per corner):
+            // a lane sums its corner's 4 channels over the run in FP32 registers, in stream order, and converts the
+            // run's sums to fixed point once, when the vertex changes. The runs and their order are fixed by the
+            // plan, so the grid gradient stays bit-reproducible. Loads go in blocks of kSB samples, the next block
+            // in flight while the current one is summed.
             constexpr int kSB = 4;
-            int vtx = -1;
-            long long acc[4] = {0, 0, 0, 0};
+            int vtx      = -1;
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};
             auto flush = [&]() {
                 if (vtx >= 0)
                 {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) atomicAdd(sq + vtx * 4 + q, (unsigned long long)acc[q]);
+                    for (int q = 0; q < 4; ++q)
+                        atomicAdd(sq + q * nv + vtx, (unsigned long long)__double2ll_rn((double)acc[q] * scale));
                 }
             };
             auto load = [&](int64_t s0, float4 (&gb)[kSB], float4 (&db)[kSB]) {
@@ -431,13 +437,13 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
                     {
                         flush();
                         vtx    = v;
-                        acc[0] = acc[1] = acc[2] = acc[3] = 0;
+                        acc[0] = acc[1] = acc[2] = acc[3] = 0.f;
                     }
                     const float4 dv = db[b];
-                    acc[0] += __double2ll_rn((double)(wk * dv.x) * scale);
-                    acc[1] += __double2ll_rn((double)(wk * dv.y) * scale);
-                    acc[2] += __double2ll_rn((double)(wk * dv.z) * scale);
-                    acc[3] += __double2ll_rn((double)(wk * dv.w) * scale);
+                    acc[0] += wk * dv.x;
+                    acc[1] += wk * dv.y;
+                    acc[2] += wk * dv.z;
+                    acc[3] += wk * dv.w;
                 }
             };
             float4 ga[kSB], gb[kSB], da[kSB], db[kSB];
@@ -459,10 +465,10 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
         for (int32_t v = threadIdx.x; v < g.G * g.G * g.G; v += blockDim.x)
         {
             float4 o;
-            o.x = (float)((double)(long long)sq[4 * v + 0] * inv);
-            o.y = (float)((double)(long long)sq[4 * v + 1] * inv);
-            o.z = (float)((double)(long long)sq[4 * v + 2] * inv);
-            o.w = (float)((double)(long long)sq[4 * v + 3] * inv);
+            o.x = (float)((double)(long long)sq[0 * nv + v] * inv);
+            o.y = (float)((double)(long long)sq[1 * nv + v] * inv);
+            o.z = (float)((double)(long long)sq[2 * nv + v] * inv);
+            o.w = (float)((double)(long long)sq[3 * nv + v] * inv);
             *reinterpret_cast<float4*>(dst + (int64_t)v * C + ch0) = o;
         }
         return;
@@ -548,8 +554,8 @@ __global__ void __launch_bounds__(256, 2) k_mfma_fwd(ProjGeom g, Plan p, const f
         __syncthreads();  // the previous chunk's readers of cx / s_seg are done
         load_chunk_ctx(g, p, q0, q1, base, nullptr, cx);
         for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x) s_seg[t] = 0.f;
-        float mn[3], mx[3];
-        node_bounds(g, g.seg_node[p.perm[q0]], mn, mx);
+        NodeBox nb;
+        node_bounds(g, g.seg_node[p.perm[q0]], nb);
         const void* ngrid = node_grid_ptr(g, grid_lowp, P::kBF16, slot);
         __syncthreads();
         const int32_t nsamp = cx.beg[nseg];
@@ -572,7 +578,7 @@ __global__ void __launch_bounds__(256, 2) k_mfma_fwd(ProjGeom g, Plan p, const f
             for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
             {
                 PairIn cur;
-                fetch_pair<P::kBF16, false>(g, cx, p0, w0, wn, base, mn, mx, ngrid, nullptr, cur);
+                fetch_pair<P::kBF16, false>(g, cx, p0, w0, wn, base, nb, ngrid, nullptr, cur);
                 tile(p0 + n, cur.xa, cur.va);
                 if (p0 + 32 < wn) tile(p0 + 32 + n, cur.xb, cur.vb);
             }
@@ -673,8 +679,8 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
         const int64_t base = p.nm_soff[q0];
         __syncthreads();
         load_chunk_ctx(g, p, q0, q1, base, grad_proj, cx);
-        float mn[3], mx[3];
-        node_bounds(g, g.seg_node[p.perm[q0]], mn, mx);
+        NodeBox nb;
+        node_bounds(g, g.seg_node[p.perm[q0]], nb);
         const void* ngrid = node_grid_ptr(g, grid_lowp, P::kBF16, slot);
         __syncthreads();
         const int32_t nsamp = cx.beg[nseg];
@@ -797,7 +803,7 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
             for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
             {
                 PairIn cur;
-                fetch_pair<P::kBF16, true>(g, cx, p0, w0, wn, base, mn, mx, ngrid, sgc, cur);
+                fetch_pair<P::kBF16, true>(g, cx, p0, w0, wn, base, nb, ngrid, sgc, cur);
                 // one copy of the tile code for both halves of the pair (smaller code, fewer live registers)
 #pragma unroll 1
                 for (int h = 0; h < 2; ++h)

@@ -286,10 +286,21 @@ __device__ __forceinline__ void gather_half(const void* node_grid, const Corners
         {
             v = *reinterpret_cast<const float4*>(static_cast<const float*>(node_grid) + c.off[k] + 4 * g);
         }
-        acc.x += v.x * c.w[k];
-        acc.y += v.y * c.w[k];
-        acc.z += v.z * c.w[k];
-        acc.w += v.w * c.w[k];
+        if constexpr (kBF16Grid)
+        {
+            // the BF16 form is not bound to the oracle's rounding: fused multiply-adds
+            acc.x = fmaf(v.x, c.w[k], acc.x);
+            acc.y = fmaf(v.y, c.w[k], acc.y);
+            acc.z = fmaf(v.z, c.w[k], acc.z);
+            acc.w = fmaf(v.w, c.w[k], acc.w);
+        }
+        else
+        {
+            acc.x += v.x * c.w[k];
+            acc.y += v.y * c.w[k];
+            acc.z += v.z * c.w[k];
+            acc.w += v.w * c.w[k];
+        }
     }
     x4[0] = acc.x;
     x4[1] = acc.y;
@@ -369,8 +380,15 @@ __device__ inline void expand_window(ChunkCtx& c, int32_t nseg, int32_t w0, int3
 }
 
 // Sample j of chunk segment t: the same operations as sample_geo on the same values, so the same bits.
+// A chunk's node box. rinv[d] is 1 / (mx[d] - mn[d]) when that extent is a power of two (every octree box is), where
+// x * rinv equals x / extent exactly (both round the same exact quotient); 0 otherwise, and then the division runs.
+struct NodeBox
+{
+    float mn[3], mx[3], rinv[3];
+};
+
 __device__ __forceinline__ Sample ctx_sample(const ProjGeom& g, const ChunkCtx& c, int32_t t, int32_t j,
-                                             const float (&mn)[3], const float (&mx)[3])
+                                             const NodeBox& nb)
 {
     Sample s;
     const int64_t ray  = c.ray[t];
@@ -401,20 +419,25 @@ __device__ __forceinline__ Sample ctx_sample(const ProjGeom& g, const ChunkCtx& 
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
-        float pd = c.o[d][t] + c.d[d][t] * tt;
-        s.uv[d]  = (pd - mn[d]) / (mx[d] - mn[d]) * 2 - 1;
+        const float pd  = c.o[d][t] + c.d[d][t] * tt;
+        const float rel = nb.rinv[d] != 0.f ? (pd - nb.mn[d]) * nb.rinv[d] : (pd - nb.mn[d]) / (nb.mx[d] - nb.mn[d]);
+        s.uv[d]         = rel * 2 - 1;
     }
     s.ray = ray;
     return s;
 }
 
-__device__ __forceinline__ void node_bounds(const ProjGeom& g, int32_t node, float (&mn)[3], float (&mx)[3])
+__device__ __forceinline__ void node_bounds(const ProjGeom& g, int32_t node, NodeBox& nb)
 {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
-        mn[d] = g.pmin[3 * node + d];
-        mx[d] = g.pmax[3 * node + d];
+        nb.mn[d]         = g.pmin[3 * node + d];
+        nb.mx[d]         = g.pmax[3 * node + d];
+        const float e    = nb.mx[d] - nb.mn[d];
+        const uint32_t b = __float_as_uint(e);
+        const bool pow2  = e > 0.f && (b & 0x7FFFFFu) == 0u && (b >> 23) > 1u && (b >> 23) < 253u;
+        nb.rinv[d]       = pow2 ? 1.0f / e : 0.f;
     }
 }
 
@@ -430,8 +453,8 @@ struct PairIn
 
 template <bool kBF16, bool kBwd>
 __device__ __forceinline__ void fetch_pair(const ProjGeom& g, const ChunkCtx& cx, int32_t p0, int32_t w0,
-                                           int32_t wn, int64_t base, const float (&mn)[3], const float (&mx)[3],
-                                           const void* ngrid, float4* sgc, PairIn& in)
+                                           int32_t wn, int64_t base, const NodeBox& nb, const void* ngrid,
+                                           float4* sgc, PairIn& in)
 {
     const int l = threadIdx.x & 63, hg = l >> 5;
     const int32_t ii = p0 + l;
@@ -440,7 +463,7 @@ __device__ __forceinline__ void fetch_pair(const ProjGeom& g, const ChunkCtx& cx
     if (ii < wn)
     {
         const int32_t t = cx.sq[ii];
-        Sample smp      = ctx_sample(g, cx, t, w0 + ii - cx.beg[t], mn, mx);
+        Sample smp      = ctx_sample(g, cx, t, w0 + ii - cx.beg[t], nb);
         const float ix = grid_coord(smp.uv[0], g.G), iy = grid_coord(smp.uv[1], g.G), iz = grid_coord(smp.uv[2], g.G);
         cr = corners_ix(ix, iy, iz, g.G, 8);
         v  = kBwd ? cx.gproj[t] * smp.w : smp.w;

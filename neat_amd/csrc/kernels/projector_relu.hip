@@ -115,8 +115,8 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
         const int64_t base = p.nm_soff[q0];
         __syncthreads();
         load_chunk_ctx(g, p, q0, q1, base, grad_proj, cx);
-        float mn[3], mx[3];
-        node_bounds(g, g.seg_node[p.perm[q0]], mn, mx);
+        NodeBox nb;
+        node_bounds(g, g.seg_node[p.perm[q0]], nb);
         const void* ngrid = node_grid_ptr(g, grid_lowp, true, slot);
         __syncthreads();
         const int32_t nsamp = cx.beg[nseg];
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
             for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
             {
                 PairIn cur;
-                fetch_pair<true, true>(g, cx, p0, w0, wn, base, mn, mx, ngrid, sgc, cur);
+                fetch_pair<true, true>(g, cx, p0, w0, wn, base, nb, ngrid, sgc, cur);
 #pragma unroll 1
                 for (int h = 0; h < 2; ++h)
                 {
@@ -353,8 +353,8 @@ __global__ void __launch_bounds__(256, 2) k_mfma_fwd_relu(ProjGeom g, Plan p, co
         __syncthreads();  // the previous chunk's readers of cx / s_seg are done (and the weights are in place)
         load_chunk_ctx(g, p, q0, q1, base, nullptr, cx);
         for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x) s_seg[t] = 0.f;
-        float mn[3], mx[3];
-        node_bounds(g, g.seg_node[p.perm[q0]], mn, mx);
+        NodeBox nb;
+        node_bounds(g, g.seg_node[p.perm[q0]], nb);
         const void* ngrid = node_grid_ptr(g, grid_lowp, true, slot);
         __syncthreads();
         const int32_t nsamp = cx.beg[nseg];
@@ -412,7 +412,7 @@ __global__ void __launch_bounds__(256, 2) k_mfma_fwd_relu(ProjGeom g, Plan p, co
             for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
             {
                 PairIn cur;
-                fetch_pair<true, false>(g, cx, p0, w0, wn, base, mn, mx, ngrid, nullptr, cur);
+                fetch_pair<true, false>(g, cx, p0, w0, wn, base, nb, ngrid, nullptr, cur);
                 tile(p0 + n, cur.xa, cur.va);
                 if (p0 + 32 < wn) tile(p0 + 32 + n, cur.xb, cur.vb);
             }
