@@ -478,27 +478,24 @@ __device__ __forceinline__ void fetch_pair(const ProjGeom& g, const ChunkCtx& cx
             cr.w[k]   = 0.f;
         }
     }
-    Corners pc;  // the other lane group's sample
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-    {
-        pc.off[k] = __shfl_xor(cr.off[k], 32);
-        pc.w[k]   = __shfl_xor(cr.w[k], 32);
-    }
-    const float pv = __shfl_xor(v, 32);
+    // the lane groups' halves swapped across (v_permlane32_swap): lanes of group 0 keep their sample as tile A and
+    // get group 1's as tile B, lanes of group 1 the other way round
     Corners ca, cb;
 #pragma unroll
     for (int k = 0; k < 8; ++k)
     {
-        ca.off[k] = hg == 0 ? cr.off[k] : pc.off[k];
-        ca.w[k]   = hg == 0 ? cr.w[k] : pc.w[k];
-        cb.off[k] = hg == 1 ? cr.off[k] : pc.off[k];
-        cb.w[k]   = hg == 1 ? cr.w[k] : pc.w[k];
+        const auto o = __builtin_amdgcn_permlane32_swap((uint32_t)cr.off[k], (uint32_t)cr.off[k], false, false);
+        ca.off[k]    = (int)o[0];
+        cb.off[k]    = (int)o[1];
+        const auto w = __builtin_amdgcn_permlane32_swap(__float_as_uint(cr.w[k]), __float_as_uint(cr.w[k]), false, false);
+        ca.w[k]      = __uint_as_float(w[0]);
+        cb.w[k]      = __uint_as_float(w[1]);
     }
+    const auto vv = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
     gather_half<kBF16>(ngrid, ca, hg, in.xa);
     gather_half<kBF16>(ngrid, cb, hg, in.xb);
-    in.va = hg == 0 ? v : pv;
-    in.vb = hg == 1 ? v : pv;
+    in.va = __uint_as_float(vv[0]);
+    in.vb = __uint_as_float(vv[1]);
 }
 
 // ---- relu BF16 backward (projector_relu.hip) ----------------------------------------------------------------------

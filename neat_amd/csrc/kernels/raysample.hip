@@ -39,20 +39,36 @@ __device__ __forceinline__ int32_t segment_count(const float mn[3], const float 
 
 // Children are visited in the order i ^ mask (bit k of mask set when the ray runs towards -axis k), a front-to-back
 // order for most rays, so segments mostly arrive sorted by entry t; the order does not change what is visited.
+//
+// Boxes are not read from the node table: the walk halves them as HyperTreeBaseImpl's constructor does
+// (HyperTree.cpp:94-129: center = (min + max) * 0.5, child i takes the upper half on axis k iff bit k of i), starting
+// from the root's box, and recovers a parent's box from a child's by doubling its extent (exact: every box of the
+// construction from [-1, 1] is dyadic). The only memory the walk reads per step is the 8 flag bytes of a sibling
+// group, fetched together when the walk enters the group.
+__device__ __forceinline__ uint64_t sibling_flags(const TreeArgs& T, int32_t parent)
+{
+    const uint8_t* f = T.flags + 8 * (int64_t)parent + 1;
+    uint64_t v       = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v |= (uint64_t)f[i] << (8 * i);
+    return v;
+}
+
 template <typename Visit>
 __device__ __forceinline__ void traverse(const TreeArgs& T, const float o[3], const float d[3], Visit&& visit)
 {
     const int32_t mask = (d[0] < 0.f ? 1 : 0) | (d[1] < 0.f ? 2 : 0) | (d[2] < 0.f ? 4 : 0);
-    int32_t n          = 0;
+    float mn[3] = {T.pmin[0], T.pmin[1], T.pmin[2]};
+    float mx[3] = {T.pmax[0], T.pmax[1], T.pmax[2]};
+    int32_t n    = 0;
+    uint32_t f   = T.flags[0];
+    uint64_t sib = 0;  // flags of n's sibling group (byte c = child c of n's parent)
     while (true)
     {
-        uint8_t f    = T.flags[n];
         bool descend = false;
         if (f)
         {
-            float mn[3] = {T.pmin[3 * n], T.pmin[3 * n + 1], T.pmin[3 * n + 2]};
-            float mx[3] = {T.pmax[3 * n], T.pmax[3 * n + 1], T.pmax[3 * n + 2]};
-            Hit h       = slab_test(mn, mx, o, d);
+            Hit h = slab_test(mn, mx, o, d);
             if (h.hit)
             {
                 // sliver filter (HyperTree.cu:274): float difference compared in double
@@ -62,13 +78,59 @@ __device__ __forceinline__ void traverse(const TreeArgs& T, const float o[3], co
         }
         if (descend)
         {
-            n = 8 * n + 1 + mask;
+            sib = sibling_flags(T, n);
+            n   = 8 * n + 1 + mask;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                const float c = (mn[k] + mx[k]) * 0.5f;
+                if ((mask >> k) & 1)
+                    mn[k] = c;
+                else
+                    mx[k] = c;
+            }
+            f = (uint32_t)(sib >> (8 * mask)) & 0xFFu;
             continue;
         }
-        // up while n is the last child in visit order, then to the next sibling
-        while (n != 0 && (((n - 1) & 7) ^ mask) == 7) n = (n - 1) >> 3;
-        if (n == 0) break;
-        n = ((n - 1) & ~7) + 1 + (((((n - 1) & 7) ^ mask) + 1) ^ mask);
+        // to the next sibling in visit order, going up while n is the last of its group
+        bool moved = false;
+        while (n != 0)
+        {
+            const int32_t c = (n - 1) & 7, i = c ^ mask;
+            // the parent's box, from n's
+            float pmn[3], pmx[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                const float e = mx[k] - mn[k];
+                pmn[k]        = ((c >> k) & 1) ? mn[k] - e : mn[k];
+                pmx[k]        = ((c >> k) & 1) ? mx[k] : mx[k] + e;
+            }
+            if (i < 7)
+            {
+                const int32_t c2 = (i + 1) ^ mask;
+                n                = ((n - 1) & ~7) + 1 + c2;
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                {
+                    const float cc = (pmn[k] + pmx[k]) * 0.5f;
+                    mn[k]          = ((c2 >> k) & 1) ? cc : pmn[k];
+                    mx[k]          = ((c2 >> k) & 1) ? pmx[k] : cc;
+                }
+                f     = (uint32_t)(sib >> (8 * c2)) & 0xFFu;
+                moved = true;
+                break;
+            }
+            n = (n - 1) >> 3;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                mn[k] = pmn[k];
+                mx[k] = pmx[k];
+            }
+            if (n != 0) sib = sibling_flags(T, (n - 1) >> 3);
+        }
+        if (!moved) break;
     }
 }
 
