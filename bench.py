@@ -52,6 +52,10 @@ def parse():
                     help="the proposal network's online training step runs every this many steps")
     ap.add_argument("--proposal-pretrain", type=int, default=200,
                     help="untimed proposal training steps (4096 rays each) before the timed region")
+    ap.add_argument("--regularizer", action="store_true",
+                    help="add the reference training step's volume regulariser (total variation of the active grids "
+                         "at loss_tv 1e-4 and the edge loss along neighbouring leaves' faces at loss_edge 1e-3, with "
+                         "its grid gradient) to every step, timed as its own stage")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference kernel for scaling HBM "
                          "byte counters) and exit")
@@ -115,6 +119,9 @@ def main():
             prop.train_step(ds, to, td, pipeline.sample(ds, to, td, jitter=False, emit=False), precision="bf16",
                             grid_bf16=grid_bf16)
         torch.cuda.synchronize()
+    if a.regularizer:
+        stage_names = stage_names + ["regularizer"]
+        active_ids = torch.from_numpy(scene["active_ids"]).to(dev)
     events = {k: [] for k in stage_names}
     gg_buf = torch.empty_like(ds.grid)
     gp_buf = torch.empty_like(ds.params)
@@ -151,6 +158,10 @@ def main():
             rec(name, 0)
             pj.backward_stage(st, gproj, gg_buf, gp_buf)
             rec(name, 1)
+        if a.regularizer:
+            rec("regularizer", 0)
+            ops.volume_regularizer(ds.pos_min, ds.pos_max, ds.flags, ds.prefix, active_ids, ds.grid, seed=i)
+            rec("regularizer", 1)
         if timed:
             for k in stage_names:
                 events[k].append(ev[k])
@@ -233,6 +244,7 @@ def main():
         "proposal_train": ("mfma", 3 * 2 * (6 * 32 + 32 * 32 + 32 * 8) * mean_samples / 128, "TFLOP/s",
                            BF16_PEAK_TFLOPS, "k_prop_bwd"),
         "loss": ("hbm", 12 * cfg.batch, "GB/s", HBM_PEAK_GBS, "k_mse"),
+        "regularizer": ("hbm", 3 * grid_bytes, "GB/s", HBM_PEAK_GBS, "k_tv_bwd"),
         "dec_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS, "k_dec_fwd"),
         "dec_bwd": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", FP32_PEAK_TFLOPS, "k_dec_bwd"),
         "gs_fwd": ("hbm", (12 + 8 + 4 * C) * mean_samples, "GB/s", HBM_PEAK_GBS, "k_gs3d_fwd"),

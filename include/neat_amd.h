@@ -317,6 +317,47 @@ int neat_proposal_bwd(neat_stream_t stream, const float* ray_origin, const float
                       const float* seg_tmin, const float* seg_tmax, int64_t num_segments, const float* params,
                       const float* target, float scale, float* grad_params, void* workspace, size_t workspace_bytes);
 
+/* ---- volume regulariser (GeometryExEx::VolumeRegularizer, geometry_ex_ex.cpp:86-160) --------------------------------
+ * Neighbour pairs of HyperTreeBaseImpl::NodeNeighborSamples (HyperTree.cu:830-940): for every active node a (in
+ * active_node_ids order), side s in 0..2, front f in 0..1 and face point sid in 0..G*G-1, the point
+ * p = local * (max - min) + min with local[s] = f, local[(s+1)%3] = (sid % G) / (G - 1.f),
+ * local[(s+2)%3] = (sid / G) / (G - 1.f) pairs a with the active node holding p + epsilon * (f ? 1 : -1) along axis s
+ * (ActiveNodeIdForGlobalPosition, HyperTree.cu:600-640), when that node exists and is not a. Each pair gives two
+ * SampleList rows (the reference's layout): rows 2k, 2k + 1 with coordinate p, node ids a and the neighbour,
+ * ray_index = row, local_index_in_ray = 0 / 1, weight = 1. Canonical order: (a, s, f, sid) ascending (the reference's
+ * follows an atomic counter). max_pairs must be at least num_active * 6 * G * G (the reference's prediction); the
+ * row arrays hold 2 * max_pairs rows; out_ray, out_local and out_weight may be NULL. *num_pairs (device) receives the
+ * number of pairs. */
+size_t neat_neighbor_pairs_workspace_bytes(int64_t num_active, int32_t grid_size);
+int neat_neighbor_pairs(neat_stream_t stream, const float* node_pos_min, const float* node_pos_max,
+                        const uint8_t* node_flags, int32_t num_nodes, const int64_t* active_node_ids,
+                        int64_t num_active, int32_t grid_size, float epsilon, int64_t max_pairs, float* out_coord,
+                        int64_t* out_node, int64_t* out_ray, int64_t* out_local, float* out_weight,
+                        int64_t* num_pairs, void* workspace, size_t workspace_bytes);
+
+/* Scratch of neat_edge_loss_fwd (num_items = pairs) and neat_tv_loss_fwd (num_items = num_active * C * G^3). */
+size_t neat_reg_workspace_bytes(int64_t num_items);
+
+/* Edge loss (geometry_ex_ex.cpp:121-134) over the features of the neighbour rows, float [2 * num_pairs, C]:
+ * loss = mean_k(mean_c |f[2k, c] - f[2k + 1, c]| * r_k) * weight * C with r_k in [0, 1) a splitmix64 hash of
+ * (seed, k) in place of torch::rand_like. *loss is FP64 on the device. The backward writes
+ * grad_features = d loss / d features * (*grad_loss) (a device scalar). */
+int neat_edge_loss_fwd(neat_stream_t stream, const float* features, int64_t num_pairs, int32_t C, uint64_t seed,
+                       double weight, double* loss, void* workspace, size_t workspace_bytes);
+int neat_edge_loss_bwd(neat_stream_t stream, const float* features, int64_t num_pairs, int32_t C, uint64_t seed,
+                       double weight, const float* grad_loss, float* grad_features);
+
+/* Total variation (TVLoss::forward without a per-batch weight, geometry_ex_ex.h:18-61, times loss_tv,
+ * geometry_ex_ex.cpp:95-103) of the grids of the given slots of a cubic grid [N, C, G, G, G] (element strides on the
+ * host, as for neat_gs3d_fwd): loss = sum over the three axes of mean |forward difference| * C * weight. The backward
+ * adds d loss / d grid * (*grad_loss) to grad_grid (same strides). */
+int neat_tv_loss_fwd(neat_stream_t stream, const float* grid, int64_t N, int32_t C, int32_t G, const int64_t* strides,
+                     const int64_t* slots, int64_t num_active, double weight, double* loss, void* workspace,
+                     size_t workspace_bytes);
+int neat_tv_loss_bwd(neat_stream_t stream, const float* grid, int64_t N, int32_t C, int32_t G, const int64_t* strides,
+                     const int64_t* slots, int64_t num_active, double weight, const float* grad_loss,
+                     float* grad_grid);
+
 /* ---- optimizer --------------------------------------------------------------------------------------------------------
  * One Adam step over n FP32 parameters in place (replaces torch::optim::Adam::step for the grid and decoder parameter
  * groups of HierarchicalNeuralGeometry::OptimizerStep, geometry.h:66-88; groups geometry_ex_ex.cpp:40-49,

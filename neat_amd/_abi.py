@@ -88,6 +88,14 @@ SIGNATURES.update({
     "neat_proposal_bwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I64, _P, _P, _F, _P, _P, _SZ]),
     "neat_adam_step": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double, _I64]),
+    "neat_neighbor_pairs_workspace_bytes": (_SZ, [_I64, _I32]),
+    "neat_neighbor_pairs": (ctypes.c_int, [_P, _P, _P, _P, _I32, _P, _I64, _I32, _F, _I64, _P, _P, _P, _P, _P, _P, _P,
+                                           _SZ]),
+    "neat_reg_workspace_bytes": (_SZ, [_I64]),
+    "neat_edge_loss_fwd": (ctypes.c_int, [_P, _P, _I64, _I32, _U64, ctypes.c_double, _P, _P, _SZ]),
+    "neat_edge_loss_bwd": (ctypes.c_int, [_P, _P, _I64, _I32, _U64, ctypes.c_double, _P, _P]),
+    "neat_tv_loss_fwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P, _I64, ctypes.c_double, _P, _P, _SZ]),
+    "neat_tv_loss_bwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P, _I64, ctypes.c_double, _P, _P]),
 })
 
 _LIB = None

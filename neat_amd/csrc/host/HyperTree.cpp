@@ -227,3 +227,42 @@ SampleList HyperTreeBaseImpl::create_samples(const RayList& rays, int max_spn, b
     }
     return list;
 }
+
+SampleList HyperTreeBaseImpl::NodeNeighborSamples(const std::vector<int>& size, double epsilon)
+{
+    check_cuda(node_position_min, "the octree");
+    TORCH_CHECK(size.size() == 3 && size[0] == size[1] && size[1] == size[2] && size[0] >= 2,
+                "NodeNeighborSamples: a cubic face resolution >= 2 is expected");
+    const auto dev     = node_position_min.device();
+    const auto L       = torch::TensorOptions().dtype(torch::kLong).device(dev);
+    const auto F       = torch::TensorOptions().dtype(torch::kFloat32).device(dev);
+    const int G        = size[0];
+    const int64_t A    = NumActiveNodes();
+    const int64_t maxp = A * 6 * (int64_t)G * G;  // the reference's prediction, in pairs
+    SampleList list;
+    list.global_coordinate  = torch::empty({2 * maxp, 3}, F);
+    list.node_id            = torch::empty({2 * maxp}, L);
+    list.ray_index          = torch::empty({2 * maxp}, L);
+    list.local_index_in_ray = torch::empty({2 * maxp}, L);
+    list.weight             = torch::empty({2 * maxp}, F);
+    list.ray_t              = torch::zeros({2 * maxp}, F);
+    auto npairs             = torch::zeros({1}, L);
+    auto ids                = active_node_ids.to(torch::kLong).contiguous();
+    const size_t wsb        = neat_neighbor_pairs_workspace_bytes(A, G);
+    auto ws = torch::empty({(int64_t)std::max<size_t>(wsb, 16)}, torch::TensorOptions().dtype(torch::kUInt8).device(dev));
+    NEAT_CALL(neat_neighbor_pairs(current_stream(), dptr<float>(node_position_min), dptr<float>(node_position_max),
+                                  node_flags.data_ptr<uint8_t>(), NumNodes(), dptr<int64_t>(ids), A, G,
+                                  (float)epsilon, maxp, dptr<float>(list.global_coordinate),
+                                  dptr<int64_t>(list.node_id), dptr<int64_t>(list.ray_index),
+                                  dptr<int64_t>(list.local_index_in_ray), dptr<float>(list.weight),
+                                  dptr<int64_t>(npairs), ws.data_ptr(), wsb));
+    // shrink to the pairs found (the reference reads its counter back too, HyperTree.cu:928-931)
+    const int64_t n = 2 * npairs.cpu().item<int64_t>();
+    list.global_coordinate  = list.global_coordinate.slice(0, 0, n);
+    list.node_id            = list.node_id.slice(0, 0, n);
+    list.ray_index          = list.ray_index.slice(0, 0, n);
+    list.local_index_in_ray = list.local_index_in_ray.slice(0, 0, n);
+    list.weight             = list.weight.slice(0, 0, n);
+    list.ray_t              = list.ray_t.slice(0, 0, n);
+    return list;
+}

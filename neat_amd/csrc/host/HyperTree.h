@@ -26,6 +26,11 @@ class HyperTreeBaseImpl : public torch::nn::Module
     // Only the canonical segment list (what the fused projector consumes); SampleList::size() is the sample count.
     SampleList CreateSegmentsForRays(const RayList& rays, int max_samples_per_node, bool interval_jitter);
 
+    // Face-point pairs between neighbouring active nodes (HyperTree.cu:830-940) as a SampleList of two rows per pair,
+    // in canonical order (active node, side, front, face point). `size` holds the per-axis face resolution (the
+    // grid size; cubic only), epsilon the offset that finds the neighbour.
+    SampleList NodeNeighborSamples(const std::vector<int>& size, double epsilon);
+
     // u = (g - min)/(max - min) * 2 - 1 for float [N, D] or [groups, group_size, D] samples, differentiable with
     // respect to global_samples (HyperTree.h:134, HyperTree.cu:116-199).
     torch::Tensor ComputeLocalSamples(torch::Tensor global_samples, torch::Tensor node_indices);

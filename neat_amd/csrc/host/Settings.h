@@ -40,6 +40,9 @@ struct MyTrainParams
     // fused projector arithmetic: "fp32" (matrix cores), "bf16" (BF16 grid copy and hidden layers, FP32
     // accumulation) or "fp32_valu" (vector ALUs)
     std::string projector_precision = "fp32";
+    // volume regulariser weights (Settings.h:223-224 in the reference)
+    double loss_tv   = 1e-4;
+    double loss_edge = 1e-3;
 };
 
 struct CombinedParams

@@ -80,6 +80,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("CreateSegmentsForRays", &HyperTreeBaseImpl::CreateSegmentsForRays)
         .def("ComputeLocalSamples", &HyperTreeBaseImpl::ComputeLocalSamples)
         .def("GlobalNodeIdToLocalActiveId", &HyperTreeBaseImpl::GlobalNodeIdToLocalActiveId)
+        .def("NodeNeighborSamples", &HyperTreeBaseImpl::NodeNeighborSamples)
         .def("ActiveNodeTensor", &HyperTreeBaseImpl::ActiveNodeTensor)
         .def("InactiveNodeIds", &HyperTreeBaseImpl::InactiveNodeIds)
         .def("NumNodes", &HyperTreeBaseImpl::NumNodes)
@@ -140,7 +141,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("lr_exex_grid_adam", &MyTrainParams::lr_exex_grid_adam)
         .def_readwrite("use_fused_projector", &MyTrainParams::use_fused_projector)
         .def_readwrite("compensated_sum", &MyTrainParams::compensated_sum)
-        .def_readwrite("projector_precision", &MyTrainParams::projector_precision);
+        .def_readwrite("projector_precision", &MyTrainParams::projector_precision)
+        .def_readwrite("loss_tv", &MyTrainParams::loss_tv)
+        .def_readwrite("loss_edge", &MyTrainParams::loss_edge);
     py::class_<CombinedParams, std::shared_ptr<CombinedParams>>(m, "CombinedParams")
         .def(py::init<>())
         .def_readwrite("net_params", &CombinedParams::net_params)
@@ -160,6 +163,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("CreateGeometryOptimizer", &GeometryExEx::CreateGeometryOptimizer)
         .def("OptimizerStep", &GeometryExEx::OptimizerStep)
         .def("UpdateLearningRate", &GeometryExEx::UpdateLearningRate)
+        .def("VolumeRegularizer", &GeometryExEx::VolumeRegularizer, py::arg("seed") = -1)
         .def_property_readonly("decoder", [](GeometryExEx& g) { return g.decoder.ptr(); })
         .def_property_readonly("grid", [](GeometryExEx& g) { return g.explicit_grid_generator->grid_data; });
 }

@@ -200,6 +200,101 @@ torch::Tensor HierarchicalNeuralGeometry::ComputeImage(SampleList all_samples, i
     return img;
 }
 
+// TVLoss::forward(grid[slots]) * weight (geometry_ex_ex.h:18-61) on neat_tv_loss_fwd / _bwd.
+struct TVLossFn : public torch::autograd::Function<TVLossFn>
+{
+    static torch::Tensor forward(torch::autograd::AutogradContext* ctx, torch::Tensor grid, torch::Tensor slots,
+                                 double weight)
+    {
+        TORCH_CHECK(grid.dim() == 5 && grid.size(2) == grid.size(3) && grid.size(3) == grid.size(4),
+                    "TV loss: a cubic grid [N, C, G, G, G] is expected");
+        const int64_t st[5] = {grid.stride(0), grid.stride(1), grid.stride(2), grid.stride(3), grid.stride(4)};
+        auto loss = torch::empty({1}, grid.options().dtype(torch::kFloat64));
+        const int64_t A = slots.size(0), C = grid.size(1), G = grid.size(2);
+        const size_t wsb = neat_reg_workspace_bytes(A * C * G * G * G);
+        auto ws = torch::empty({(int64_t)std::max<size_t>(wsb, 16)}, grid.options().dtype(torch::kUInt8));
+        NEAT_CALL(neat_tv_loss_fwd(current_stream(), dptr<float>(grid), grid.size(0), (int32_t)C, (int32_t)G, st,
+                                   dptr<int64_t>(slots), A, weight, dptr<double>(loss), ws.data_ptr(), wsb));
+        ctx->save_for_backward({grid, slots});
+        ctx->saved_data["weight"] = weight;
+        return loss.to(torch::kFloat32).reshape({});
+    }
+
+    static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
+                                                   torch::autograd::variable_list grad_out)
+    {
+        auto saved  = ctx->get_saved_variables();
+        auto grid   = saved[0], slots = saved[1];
+        auto gl     = grad_out[0].to(torch::kFloat32).reshape({1}).contiguous();
+        auto gg     = torch::zeros_like(grid);
+        const int64_t st[5] = {grid.stride(0), grid.stride(1), grid.stride(2), grid.stride(3), grid.stride(4)};
+        TORCH_CHECK(gg.strides() == grid.strides(), "TV loss: gradient layout");
+        NEAT_CALL(neat_tv_loss_bwd(current_stream(), dptr<float>(grid), grid.size(0), (int32_t)grid.size(1),
+                                   (int32_t)grid.size(2), st, dptr<int64_t>(slots), slots.size(0),
+                                   ctx->saved_data["weight"].toDouble(), dptr<float>(gl), dptr<float>(gg)));
+        return {gg, torch::Tensor(), torch::Tensor()};
+    }
+};
+
+// mean_k(mean_c |f[2k] - f[2k+1]| * r_k) * weight * C (geometry_ex_ex.cpp:121-134) on neat_edge_loss_fwd / _bwd.
+struct EdgeLossFn : public torch::autograd::Function<EdgeLossFn>
+{
+    static torch::Tensor forward(torch::autograd::AutogradContext* ctx, torch::Tensor features, int64_t seed,
+                                 double weight)
+    {
+        auto f          = features.contiguous();
+        const int64_t P = f.size(0) / 2;
+        auto loss       = torch::empty({1}, f.options().dtype(torch::kFloat64));
+        const size_t wsb = neat_reg_workspace_bytes(P);
+        auto ws = torch::empty({(int64_t)std::max<size_t>(wsb, 16)}, f.options().dtype(torch::kUInt8));
+        NEAT_CALL(neat_edge_loss_fwd(current_stream(), dptr<float>(f), P, (int32_t)f.size(1), (uint64_t)seed, weight,
+                                     dptr<double>(loss), ws.data_ptr(), wsb));
+        ctx->save_for_backward({f});
+        ctx->saved_data["seed"]   = seed;
+        ctx->saved_data["weight"] = weight;
+        return loss.to(torch::kFloat32).reshape({});
+    }
+
+    static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
+                                                   torch::autograd::variable_list grad_out)
+    {
+        auto f  = ctx->get_saved_variables()[0];
+        auto gl = grad_out[0].to(torch::kFloat32).reshape({1}).contiguous();
+        auto gf = torch::empty_like(f);
+        NEAT_CALL(neat_edge_loss_bwd(current_stream(), dptr<float>(f), f.size(0) / 2, (int32_t)f.size(1),
+                                     (uint64_t)ctx->saved_data["seed"].toInt(), ctx->saved_data["weight"].toDouble(),
+                                     dptr<float>(gl), dptr<float>(gf)));
+        return {gf, torch::Tensor(), torch::Tensor()};
+    }
+};
+
+torch::Tensor GeometryExEx::VolumeRegularizer(int64_t seed)
+{
+    auto& tp  = params->train_params;
+    auto grid = explicit_grid_generator->grid_data;
+    torch::Tensor loss;
+    if (tp.loss_tv > 0)
+    {
+        auto slots = tree->GlobalNodeIdToLocalActiveId(tree->ActiveNodeTensor()).to(torch::kLong).contiguous();
+        loss       = TVLossFn::apply(grid, slots, tp.loss_tv);
+    }
+    if (tp.loss_edge > 0)
+    {
+        const int G = params->net_params.grid_size;
+        SampleList nb = tree->NodeNeighborSamples({G, G, G}, 0.001);
+        if (nb.size() > 0)
+        {
+            auto local_samples   = tree->ComputeLocalSamples(nb.global_coordinate, nb.node_id);
+            auto local_node_id   = tree->GlobalNodeIdToLocalActiveId(nb.node_id);
+            auto neural_features = IndirectGridSample3D(grid, local_node_id, local_samples);
+            if (seed < 0) seed = torch::randint(0, (int64_t)1 << 62, {1}, torch::kLong).item<int64_t>();
+            auto edge = EdgeLossFn::apply(neural_features, seed, tp.loss_edge);
+            loss      = loss.defined() ? loss + edge : edge;
+        }
+    }
+    return loss;
+}
+
 GeometryExEx::GeometryExEx(int num_channels, int D, HyperTreeBase tree, std::shared_ptr<CombinedParams> params)
     : HierarchicalNeuralGeometry(num_channels, D, params, tree)
 {

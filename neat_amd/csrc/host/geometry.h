@@ -81,5 +81,10 @@ class GeometryExEx : public HierarchicalNeuralGeometry
     // the grid group at train_params.lr_exex_grid_adam (geometry_ex_ex.cpp:40-49)
     void AddParametersToOptimizer() override;
 
+    // total variation of the active grids times loss_tv, plus the edge loss along the faces of neighbouring active
+    // nodes times loss_edge (geometry_ex_ex.cpp:86-160); differentiable with respect to the grid. seed < 0 draws the
+    // per-pair factors' seed from torch's default generator (the reference draws torch::rand_like).
+    torch::Tensor VolumeRegularizer(int64_t seed = -1);
+
     ExplicitFeatureGrid explicit_grid_generator = nullptr;
 };

@@ -311,3 +311,94 @@ def proposal_gate(segs, seg_count_out, seg_cdf, prefix, num_slots, roughness, ta
     K = segs["seg_count"].shape[0]
     call("neat_proposal_gate", stream_ptr(stream), ptr(segs["seg_node"]), ptr(segs["seg_count"]), K, ptr(prefix),
          prefix.shape[0], int(num_slots), ptr(roughness), float(tau), ptr(seg_cdf), ptr(seg_count_out))
+
+
+# ---- volume regulariser (GeometryExEx::VolumeRegularizer, geometry_ex_ex.cpp:86-160) ---------------------------------
+def workspace(nbytes, device):
+    return torch.empty(max(int(nbytes), 16), dtype=torch.uint8, device=device)
+
+
+def neighbor_samples(pos_min, pos_max, flags, active_ids, grid_size, epsilon=0.001, stream=None):
+    """HyperTreeBaseImpl::NodeNeighborSamples (HyperTree.cu:830-940) in canonical order: SampleList columns of the
+    neighbour rows (two per pair: coordinate, node id, ray index, local index, weight), shrunk to the pairs found."""
+    A = int(active_ids.shape[0])
+    dev = pos_min.device
+    max_pairs = A * 6 * grid_size * grid_size
+    rows = 2 * max_pairs
+    coord = torch.empty(rows, 3, dtype=_F32, device=dev)
+    node = torch.empty(rows, dtype=torch.int64, device=dev)
+    ray = torch.empty(rows, dtype=torch.int64, device=dev)
+    local = torch.empty(rows, dtype=torch.int64, device=dev)
+    weight = torch.empty(rows, dtype=_F32, device=dev)
+    npairs = torch.zeros(1, dtype=torch.int64, device=dev)
+    ws = workspace(_abi.lib().neat_neighbor_pairs_workspace_bytes(A, grid_size), dev)
+    call("neat_neighbor_pairs", stream_ptr(stream), ptr(pos_min), ptr(pos_max), ptr(flags), pos_min.shape[0],
+         ptr(active_ids.contiguous()), A, grid_size, float(epsilon), max_pairs, ptr(coord), ptr(node), ptr(ray),
+         ptr(local), ptr(weight), ptr(npairs), ptr(ws), ws.numel())
+    n = 2 * int(npairs.item())
+    return dict(global_coordinate=coord[:n], node_id=node[:n], ray_index=ray[:n], local_index_in_ray=local[:n],
+                weight=weight[:n], num_pairs=n // 2)
+
+
+def edge_loss_fwd(features, seed, weight, stream=None):
+    """mean over pairs of mean_c |f[2k] - f[2k+1]| * r_k, times weight * C; FP64 scalar tensor."""
+    P, C = features.shape[0] // 2, features.shape[1]
+    out = torch.empty(1, dtype=torch.float64, device=features.device)
+    ws = workspace(_abi.lib().neat_reg_workspace_bytes(P), features.device)
+    call("neat_edge_loss_fwd", stream_ptr(stream), ptr(features.contiguous()), P, C, int(seed), float(weight),
+         ptr(out), ptr(ws), ws.numel())
+    return out
+
+
+def edge_loss_bwd(features, seed, weight, grad_loss, stream=None):
+    P, C = features.shape[0] // 2, features.shape[1]
+    g = torch.empty_like(features)
+    call("neat_edge_loss_bwd", stream_ptr(stream), ptr(features.contiguous()), P, C, int(seed), float(weight),
+         ptr(grad_loss.to(_F32).contiguous()), ptr(g))
+    return g
+
+
+def tv_loss_fwd(grid, slots, weight, stream=None):
+    """Total variation of grid[slots] (TVLoss::forward) times weight; FP64 scalar tensor."""
+    N, C, G = grid.shape[0], grid.shape[1], grid.shape[2]
+    st = _strides(grid)
+    A = int(slots.shape[0])
+    out = torch.empty(1, dtype=torch.float64, device=grid.device)
+    ws = workspace(_abi.lib().neat_reg_workspace_bytes(A * C * G * G * G), grid.device)
+    call("neat_tv_loss_fwd", stream_ptr(stream), ptr(grid), N, C, G, st.ctypes.data, ptr(slots.contiguous()), A,
+         float(weight), ptr(out), ptr(ws), ws.numel())
+    return out
+
+
+def tv_loss_bwd(grid, slots, weight, grad_loss, grad_grid, stream=None):
+    """Adds d tv / d grid * grad_loss to grad_grid (same strides as grid)."""
+    N, C, G = grid.shape[0], grid.shape[1], grid.shape[2]
+    assert grad_grid.stride() == grid.stride()
+    st = _strides(grid)
+    call("neat_tv_loss_bwd", stream_ptr(stream), ptr(grid), N, C, G, st.ctypes.data, ptr(slots.contiguous()),
+         int(slots.shape[0]), float(weight), ptr(grad_loss.to(_F32).contiguous()), ptr(grad_grid))
+
+
+def volume_regularizer(pos_min, pos_max, flags, prefix, active_ids, grid, loss_tv=1e-4, loss_edge=1e-3, seed=0,
+                       epsilon=0.001, stream=None):
+    """GeometryExEx::VolumeRegularizer's value and grid gradient: tv (loss_tv > 0) plus the edge loss (loss_edge > 0)
+    along the faces of neighbouring active nodes. Returns (loss FP64 scalar tensor, grad_grid)."""
+    G = grid.shape[2]
+    gg = torch.zeros_like(grid)
+    one = torch.ones(1, dtype=_F32, device=grid.device)
+    loss = torch.zeros(1, dtype=torch.float64, device=grid.device)
+    if loss_tv > 0:
+        slots = prefix[active_ids]
+        loss += tv_loss_fwd(grid, slots, loss_tv, stream)
+        tv_loss_bwd(grid, slots, loss_tv, one, gg, stream)
+    if loss_edge > 0:
+        smp = neighbor_samples(pos_min, pos_max, flags, active_ids, G, epsilon, stream)
+        if smp["num_pairs"] > 0:
+            uv = local_samples(smp["global_coordinate"], smp["node_id"], pos_min, pos_max, stream)
+            slot = node_to_slot(smp["node_id"], prefix, stream)
+            f = gs3d_fwd(grid, slot, uv, stream)
+            loss += edge_loss_fwd(f, seed, loss_edge, stream)
+            gf = edge_loss_bwd(f, seed, loss_edge, one, stream)
+            g2, _ = gs3d_bwd(grid, slot, uv, gf, want_uv=False, stream=stream)
+            gg += g2
+    return loss, gg

@@ -175,3 +175,30 @@ def test_compute_image_precision_forms(H, oracle, precision):
         assert e[0] <= 1e-3 and e[1] <= 1e-1 and e[2] <= 1e-1
     else:
         assert max(e) <= 1e-5
+
+
+def test_volume_regularizer_module_vs_oracle(H, oracle):
+    """GeometryExEx::VolumeRegularizer (geometry_ex_ex.cpp:86-160): loss and grid gradient through autograd against
+    the oracle's composition (neighbour pairs, local coordinates, grid sample, edge loss; total variation)."""
+    from oracle import regularizer as R
+    print("case: GeometryExEx.VolumeRegularizer C1", flush=True)
+    cfg = W.C1
+    sc, _, _, geo = make_geometry(H, cfg, seed=5)
+    p = geo.grid
+    tree = sc["tree"]
+    ids, prefix = sc["active_ids"], sc["prefix"]
+    seed = 1234
+    loss = geo.VolumeRegularizer(seed)
+    loss.backward()
+    gg = p.grad.detach().cpu().numpy()
+    grid = p.detach().cpu().numpy()
+    coord, node = R.neighbor_pairs(tree, sc["active"], ids, cfg.grid)
+    uv = oracle.local_samples(tree, coord, node)
+    f = oracle.grid_sample_fwd(grid.astype(np.float64), prefix[node], uv.astype(np.float64))
+    el, ef = R.edge_loss(f, seed, 1e-3)
+    eg, _ = oracle.grid_sample_bwd(grid.astype(np.float64), prefix[node], uv.astype(np.float64), ef)
+    tl, tg = R.tv_loss(grid, prefix[ids], 1e-4)
+    e_loss = abs(loss.item() - (el + tl)) / abs(el + tl)
+    e_grad = rel(gg, eg + tg)
+    print("VolumeRegularizer loss", loss.item(), el + tl, "rel", e_loss, "grad rel", e_grad)
+    assert e_loss <= 1e-5 and e_grad <= 1e-5
