@@ -358,6 +358,19 @@ int neat_tv_loss_bwd(neat_stream_t stream, const float* grid, int64_t N, int32_t
                      const int64_t* slots, int64_t num_active, double weight, const float* grad_loss,
                      float* grad_grid);
 
+/* ---- per-node loss of the evaluation pass --------------------------------------------------------------------------------
+ * HierarchicalNeuralGeometry::AccumulateSampleLossPerNode (geometry.cpp:38-58): loss_sum[node] = sum over the node's
+ * samples of per_ray_loss[ray] * weight, weight_sum[node] = sum of the weights, float [num_nodes], over the canonical
+ * segment list of a uniformly sampled batch (neat_raysample_segments with the same jitter / seed as the emitted
+ * samples; the weights are recomputed bit-identical to neat_raysample_emit's). Sums: FP32 per segment in j order,
+ * then FP64 per node over its segments in canonical order (the reference's scatter_add follows atomic order). */
+size_t neat_node_loss_workspace_bytes(int64_t num_segments, int32_t num_nodes);
+int neat_node_loss_accumulate(neat_stream_t stream, const int64_t* ray_seg_offset, int64_t num_rays,
+                              const int32_t* seg_node, const int32_t* seg_ray, const int32_t* seg_count,
+                              const float* seg_tmin, const float* seg_tmax, int64_t num_segments, int32_t jitter,
+                              uint64_t seed, const float* per_ray_loss, int32_t num_nodes, float* loss_sum,
+                              float* weight_sum, void* workspace, size_t workspace_bytes);
+
 /* ---- optimizer --------------------------------------------------------------------------------------------------------
  * One Adam step over n FP32 parameters in place (replaces torch::optim::Adam::step for the grid and decoder parameter
  * groups of HierarchicalNeuralGeometry::OptimizerStep, geometry.h:66-88; groups geometry_ex_ex.cpp:40-49,

@@ -96,6 +96,9 @@ SIGNATURES.update({
     "neat_edge_loss_bwd": (ctypes.c_int, [_P, _P, _I64, _I32, _U64, ctypes.c_double, _P, _P]),
     "neat_tv_loss_fwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P, _I64, ctypes.c_double, _P, _P, _SZ]),
     "neat_tv_loss_bwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P, _I64, ctypes.c_double, _P, _P]),
+    "neat_node_loss_workspace_bytes": (_SZ, [_I64, _I32]),
+    "neat_node_loss_accumulate": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P, _P, _P, _I64, _I32, _U64, _P, _I32, _P, _P,
+                                                 _P, _SZ]),
 })
 
 _LIB = None

@@ -266,3 +266,28 @@ SampleList HyperTreeBaseImpl::NodeNeighborSamples(const std::vector<int>& size, 
     list.ray_t              = list.ray_t.slice(0, 0, n);
     return list;
 }
+
+void HyperTreeBaseImpl::SetErrorForActiveNodes(torch::Tensor error, std::string strategy)
+{
+    // elementwise over [num_nodes], the reference's tensor expressions (control-plane bookkeeping, not a hot path)
+    TORCH_CHECK(error.sizes() == node_error.sizes(), "SetErrorForActiveNodes: error must be [num_nodes]");
+    TORCH_CHECK(error.device() == node_error.device(), "SetErrorForActiveNodes: device mismatch");
+    auto active_float   = node_active.to(torch::kFloat32);
+    auto inactive       = 1 - active_float;
+    auto invalid_errors = (node_error < 0).to(torch::kFloat32);
+    error               = active_float * error;
+    if (strategy == "override")
+    {
+        node_error.set_data(node_error * inactive + error);
+    }
+    else if (strategy == "min")
+    {
+        error               = error + inactive * 1000000;
+        auto filtered_error = node_error + (invalid_errors * active_float) * 1000000;
+        node_error.set_data(torch::min(filtered_error, error));
+    }
+    else
+    {
+        TORCH_CHECK(false, "SetErrorForActiveNodes: unknown strategy '", strategy, "' (override, min)");
+    }
+}

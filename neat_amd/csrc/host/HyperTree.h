@@ -35,6 +35,10 @@ class HyperTreeBaseImpl : public torch::nn::Module
     // respect to global_samples (HyperTree.h:134, HyperTree.cu:116-199).
     torch::Tensor ComputeLocalSamples(torch::Tensor global_samples, torch::Tensor node_indices);
 
+    // node_error update from a per-node error (HyperTree.cpp:225-265): "override" replaces the active nodes' errors,
+    // "min" keeps the smaller of the old (where already set) and the new one; inactive nodes keep theirs.
+    void SetErrorForActiveNodes(torch::Tensor error, std::string strategy = "override");
+
     // node id -> grid slot (HyperTree.h:183-187).
     torch::Tensor GlobalNodeIdToLocalActiveId(torch::Tensor node_id);
 

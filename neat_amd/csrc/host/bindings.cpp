@@ -81,6 +81,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("ComputeLocalSamples", &HyperTreeBaseImpl::ComputeLocalSamples)
         .def("GlobalNodeIdToLocalActiveId", &HyperTreeBaseImpl::GlobalNodeIdToLocalActiveId)
         .def("NodeNeighborSamples", &HyperTreeBaseImpl::NodeNeighborSamples)
+        .def("SetErrorForActiveNodes", &HyperTreeBaseImpl::SetErrorForActiveNodes, py::arg("error"),
+             py::arg("strategy") = "override")
+        .def_readonly("node_error", &HyperTreeBaseImpl::node_error)
         .def("ActiveNodeTensor", &HyperTreeBaseImpl::ActiveNodeTensor)
         .def("InactiveNodeIds", &HyperTreeBaseImpl::InactiveNodeIds)
         .def("NumNodes", &HyperTreeBaseImpl::NumNodes)
@@ -164,6 +167,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("OptimizerStep", &GeometryExEx::OptimizerStep)
         .def("UpdateLearningRate", &GeometryExEx::UpdateLearningRate)
         .def("VolumeRegularizer", &GeometryExEx::VolumeRegularizer, py::arg("seed") = -1)
+        .def("AccumulateSampleLossPerNode", &GeometryExEx::AccumulateSampleLossPerNode)
         .def_property_readonly("decoder", [](GeometryExEx& g) { return g.decoder.ptr(); })
         .def_property_readonly("grid", [](GeometryExEx& g) { return g.explicit_grid_generator->grid_data; });
 }

@@ -295,6 +295,30 @@ torch::Tensor GeometryExEx::VolumeRegularizer(int64_t seed)
     return loss;
 }
 
+std::pair<torch::Tensor, torch::Tensor> HierarchicalNeuralGeometry::AccumulateSampleLossPerNode(
+    const SampleList& combined_samples, torch::Tensor per_ray_loss)
+{
+    const SegmentList& seg = combined_samples.segments;
+    TORCH_CHECK(seg.defined(), "AccumulateSampleLossPerNode: the SampleList has no segment list");
+    check_cuda(per_ray_loss, "per_ray_loss");
+    // [num_rays] (geometry.cpp:42)
+    auto loss = (per_ray_loss.dim() == 2 ? torch::mean(per_ray_loss, 0) : per_ray_loss).detach().to(torch::kFloat32)
+                    .contiguous();
+    const int64_t R = seg.ray_seg_offset.size(0) - 1;
+    TORCH_CHECK(loss.numel() == R, "AccumulateSampleLossPerNode: ", loss.numel(), " losses for ", R, " rays");
+    const int32_t nn = (int32_t)tree->NumNodes();
+    auto F           = loss.options();
+    auto ls = torch::empty({nn}, F), ws_ = torch::empty({nn}, F);
+    const size_t wsb = neat_node_loss_workspace_bytes(seg.size(), nn);
+    auto ws = torch::empty({(int64_t)std::max<size_t>(wsb, 16)}, F.dtype(torch::kUInt8));
+    NEAT_CALL(neat_node_loss_accumulate(current_stream(), dptr<int64_t>(seg.ray_seg_offset), R,
+                                        dptr<int32_t>(seg.node_id), dptr<int32_t>(seg.ray), dptr<int32_t>(seg.count),
+                                        dptr<float>(seg.tmin), dptr<float>(seg.tmax), seg.size(), seg.jitter ? 1 : 0,
+                                        seg.seed, dptr<float>(loss), nn, dptr<float>(ls), dptr<float>(ws_),
+                                        ws.data_ptr(), wsb));
+    return {ls, ws_};
+}
+
 GeometryExEx::GeometryExEx(int num_channels, int D, HyperTreeBase tree, std::shared_ptr<CombinedParams> params)
     : HierarchicalNeuralGeometry(num_channels, D, params, tree)
 {

@@ -53,6 +53,12 @@ class HierarchicalNeuralGeometry : public NeuralGeometry
     // The fused projector for a SampleList with segments, or an undefined tensor when not covered.
     virtual torch::Tensor ComputeImageFused(const SampleList& samples, int num_pixels) { return {}; }
 
+    // Per-node sums of the samples' per-ray loss times weight, and of the weights (geometry.cpp:38-58), float
+    // [num_nodes] each; per_ray_loss is [channels, num_rays] (averaged over channels). Needs the samples' segment
+    // list (CreateSamplesForRays keeps it in SampleList::segments).
+    std::pair<torch::Tensor, torch::Tensor> AccumulateSampleLossPerNode(const SampleList& combined_samples,
+                                                                        torch::Tensor per_ray_loss);
+
     // Adam groups as the reference creates them (geometry.h:46-56, geometry.cpp:87-97): the decoder in its own
     // optimizer at net_params.decoder_lr; subclasses add their parameters to optimizer_adam.
     void CreateGeometryOptimizer();

@@ -402,3 +402,20 @@ def volume_regularizer(pos_min, pos_max, flags, prefix, active_ids, grid, loss_t
             g2, _ = gs3d_bwd(grid, slot, uv, gf, want_uv=False, stream=stream)
             gg += g2
     return loss, gg
+
+
+# ---- per-node loss of the evaluation pass ------------------------------------------------------------------------------
+def node_loss_accumulate(segs, per_ray_loss, num_nodes, jitter=False, seed=0, stream=None):
+    """AccumulateSampleLossPerNode (geometry.cpp:38-58) over a canonical segment list (neat_node_loss_accumulate):
+    (loss_sum, weight_sum), float [num_nodes]."""
+    dev = per_ray_loss.device
+    K = int(segs["seg_node"].shape[0])
+    R = int(segs["ray_seg_offset"].shape[0]) - 1
+    ls = torch.empty(num_nodes, dtype=_F32, device=dev)
+    wsum = torch.empty(num_nodes, dtype=_F32, device=dev)
+    ws = workspace(_abi.lib().neat_node_loss_workspace_bytes(K, num_nodes), dev)
+    call("neat_node_loss_accumulate", stream_ptr(stream), ptr(segs["ray_seg_offset"]), R, ptr(segs["seg_node"]),
+         ptr(segs["seg_ray"]), ptr(segs["seg_count"]), ptr(segs["seg_tmin"]), ptr(segs["seg_tmax"]), K,
+         int(bool(jitter)), int(seed), ptr(per_ray_loss.to(_F32).contiguous()), num_nodes, ptr(ls), ptr(wsum),
+         ptr(ws), ws.numel())
+    return ls, wsum

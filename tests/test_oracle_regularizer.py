@@ -80,3 +80,54 @@ def test_neighbor_pairs_join_touching_boxes():
     # canonical order: the first node of the pairs follows active_node_ids
     order = np.searchsorted(ids, a)
     assert np.all(np.diff(order) >= 0)
+
+
+def test_node_loss_weight_sums_telescope(oracle):
+    """AccumulateSampleLossPerNode's weight sums equal every node's total segment length."""
+    from oracle import nodeloss as NL
+    tree, active, ids, _ = tree_scene(2)
+    o, d = W.cone_beam_rays(2, 16, 16)
+    smp = oracle.raysample(tree, ids, o, d, 8, jitter=True, seed=3, emit=True, nthreads=1)
+    loss = np.random.default_rng(0).uniform(size=o.shape[0])
+    ls, ws = NL.accumulate(smp, loss, tree["pos_min"].shape[0])
+    seglen = np.zeros_like(ws)
+    np.add.at(seglen, smp["seg_node"], (smp["seg_tmax"].astype(np.float64) - smp["seg_tmin"]))
+    assert np.allclose(ws, seglen, rtol=1e-5, atol=1e-6)
+    assert ws[active == 0].sum() == 0 and ls.sum() > 0
+
+
+def test_node_loss_indicator_rays_give_segment_lengths(oracle):
+    """Linearity in the per-ray loss: with loss = 1 on one ray and 0 elsewhere, a node's loss sum is that ray's
+    segment length inside the node (computed from the segment list, not from the samples' weights)."""
+    from oracle import nodeloss as NL
+    tree, active, ids, _ = tree_scene(3, adaptive=40, seed=1)
+    o, d = W.cone_beam_rays(3, 8, 8)
+    smp = oracle.raysample(tree, ids, o, d, 6, jitter=False, seed=0, emit=True, nthreads=1)
+    nn = tree["pos_min"].shape[0]
+    R_ = o.shape[0]
+    seg_ray = smp["seg_ray"].astype(np.int64)
+    for r in np.unique(seg_ray)[:: max(1, len(np.unique(seg_ray)) // 12)]:
+        loss = np.zeros(R_)
+        loss[r] = 1.0
+        ls, _ = NL.accumulate(smp, loss, nn)
+        want = np.zeros(nn)
+        m = seg_ray == r
+        np.add.at(want, smp["seg_node"][m], smp["seg_tmax"][m].astype(np.float64) - smp["seg_tmin"][m])
+        assert np.allclose(ls, want, rtol=1e-5, atol=1e-6)
+    # a constant loss c scales the weight sums
+    ls, ws = NL.accumulate(smp, np.full(R_, 2.5), nn)
+    assert np.allclose(ls, 2.5 * ws, rtol=1e-12, atol=0)
+
+
+def test_set_error_semantics():
+    """SetErrorForActiveNodes: "override" replaces active nodes' error and keeps inactive ones; "min" keeps the
+    smaller of old and new on active nodes whose old error is valid (>= 0), takes the new one where the old is
+    negative, and keeps inactive nodes."""
+    from oracle import nodeloss as NL
+    active = np.array([1, 0, 1, 1, 0, 1], np.int32)
+    old = np.array([0.5, 7.0, -1.0, 2.0, -3.0, 0.25], np.float32)
+    new = np.array([0.75, 9.0, 4.0, 1.5, 6.0, 0.25], np.float32)
+    ov = NL.set_error(old, active, new, "override")
+    assert np.array_equal(ov, np.array([0.75, 7.0, 4.0, 1.5, -3.0, 0.25], np.float32))
+    mn = NL.set_error(old, active, new, "min")
+    assert np.array_equal(mn, np.array([0.5, 7.0, 4.0, 1.5, -3.0, 0.25], np.float32))
