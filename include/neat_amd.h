@@ -335,8 +335,9 @@ int neat_neighbor_pairs(neat_stream_t stream, const float* node_pos_min, const f
                         int64_t* out_node, int64_t* out_ray, int64_t* out_local, float* out_weight,
                         int64_t* num_pairs, void* workspace, size_t workspace_bytes);
 
-/* Scratch of neat_edge_loss_fwd (num_items = pairs) and neat_tv_loss_fwd (num_items = num_active * C * G^3). */
+/* Scratch of neat_edge_loss_fwd (num_items = pairs) and of neat_tv_loss_fwd. */
 size_t neat_reg_workspace_bytes(int64_t num_items);
+size_t neat_tv_workspace_bytes(int64_t num_active, int32_t C, int32_t G);
 
 /* Edge loss (geometry_ex_ex.cpp:121-134) over the features of the neighbour rows, float [2 * num_pairs, C]:
  * loss = mean_k(mean_c |f[2k, c] - f[2k + 1, c]| * r_k) * weight * C with r_k in [0, 1) a splitmix64 hash of
@@ -350,13 +351,35 @@ int neat_edge_loss_bwd(neat_stream_t stream, const float* features, int64_t num_
 /* Total variation (TVLoss::forward without a per-batch weight, geometry_ex_ex.h:18-61, times loss_tv,
  * geometry_ex_ex.cpp:95-103) of the grids of the given slots of a cubic grid [N, C, G, G, G] (element strides on the
  * host, as for neat_gs3d_fwd): loss = sum over the three axes of mean |forward difference| * C * weight. The backward
- * adds d loss / d grid * (*grad_loss) to grad_grid (same strides). */
+ * adds d loss / d grid * (*grad_loss) to grad_grid (same strides). Workspace: neat_tv_workspace_bytes(num_active,
+ * C, G); C * G^3 must be below 2^24. */
 int neat_tv_loss_fwd(neat_stream_t stream, const float* grid, int64_t N, int32_t C, int32_t G, const int64_t* strides,
                      const int64_t* slots, int64_t num_active, double weight, double* loss, void* workspace,
                      size_t workspace_bytes);
 int neat_tv_loss_bwd(neat_stream_t stream, const float* grid, int64_t N, int32_t C, int32_t G, const int64_t* strides,
                      const int64_t* slots, int64_t num_active, double weight, const float* grad_loss,
                      float* grad_grid);
+/* Both in one pass over the grid: loss as neat_tv_loss_fwd, and d loss / d grid * (*grad_loss) written into the
+ * given slots of grad_grid (overwrite != 0: entries of other slots are left untouched, the caller clears them) or
+ * added to it (overwrite == 0). Same workspace as neat_tv_loss_fwd. No counterpart in the reference (its autograd runs
+ * the two halves separately); for callers that want the value and the gradient together. */
+int neat_tv_loss_fwd_bwd(neat_stream_t stream, const float* grid, int64_t N, int32_t C, int32_t G,
+                         const int64_t* strides, const int64_t* slots, int64_t num_active, double weight, double* loss,
+                         const float* grad_loss, float* grad_grid, int32_t overwrite, void* workspace,
+                         size_t workspace_bytes);
+
+/* The edge loss of GeometryExEx::VolumeRegularizer (geometry_ex_ex.cpp:114-134) in one pass over the candidate face
+ * points, replacing the chain NodeNeighborSamples -> ComputeLocalSamples -> IndirectGridSample3D -> edge loss (and its
+ * backward through the grid sample) without materialising the neighbour rows or their features. Same pairs, pair
+ * order, random factor and per-feature gradient as that chain. node_slot[node] = the node's grid slot (N for none);
+ * grid [N, C, G, G, G] with element strides. loss (nullable) = the edge loss (0 without pairs); grad_grid (nullable,
+ * needs grad_loss) gets d loss / d grid * (*grad_loss) added with float atomics. */
+size_t neat_edge_loss_direct_workspace_bytes(int64_t num_active, int32_t grid_size);
+int neat_edge_loss_direct(neat_stream_t stream, const float* node_pos_min, const float* node_pos_max,
+                          const uint8_t* node_flags, int32_t num_nodes, const int64_t* active_node_ids,
+                          int64_t num_active, const int64_t* node_slot, const float* grid, int64_t N, int32_t C,
+                          int32_t G, const int64_t* strides, float epsilon, uint64_t seed, double weight, double* loss,
+                          const float* grad_loss, float* grad_grid, void* workspace, size_t workspace_bytes);
 
 /* ---- per-node loss of the evaluation pass --------------------------------------------------------------------------------
  * HierarchicalNeuralGeometry::AccumulateSampleLossPerNode (geometry.cpp:38-58): loss_sum[node] = sum over the node's

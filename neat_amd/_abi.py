@@ -92,10 +92,16 @@ SIGNATURES.update({
     "neat_neighbor_pairs": (ctypes.c_int, [_P, _P, _P, _P, _I32, _P, _I64, _I32, _F, _I64, _P, _P, _P, _P, _P, _P, _P,
                                            _SZ]),
     "neat_reg_workspace_bytes": (_SZ, [_I64]),
+    "neat_tv_workspace_bytes": (_SZ, [_I64, _I32, _I32]),
+    "neat_tv_loss_fwd_bwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P, _I64, ctypes.c_double, _P, _P, _P,
+                                            _I32, _P, _SZ]),
     "neat_edge_loss_fwd": (ctypes.c_int, [_P, _P, _I64, _I32, _U64, ctypes.c_double, _P, _P, _SZ]),
     "neat_edge_loss_bwd": (ctypes.c_int, [_P, _P, _I64, _I32, _U64, ctypes.c_double, _P, _P]),
     "neat_tv_loss_fwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P, _I64, ctypes.c_double, _P, _P, _SZ]),
     "neat_tv_loss_bwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P, _I64, ctypes.c_double, _P, _P]),
+    "neat_edge_loss_direct_workspace_bytes": (_SZ, [_I64, _I32]),
+    "neat_edge_loss_direct": (ctypes.c_int, [_P, _P, _P, _P, _I32, _P, _I64, _P, _P, _I64, _I32, _I32, _P, _F, _U64,
+                                             ctypes.c_double, _P, _P, _P, _P, _SZ]),
     "neat_node_loss_workspace_bytes": (_SZ, [_I64, _I32]),
     "neat_node_loss_accumulate": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P, _P, _P, _I64, _I32, _U64, _P, _I32, _P, _P,
                                                  _P, _SZ]),

@@ -167,6 +167,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("OptimizerStep", &GeometryExEx::OptimizerStep)
         .def("UpdateLearningRate", &GeometryExEx::UpdateLearningRate)
         .def("VolumeRegularizer", &GeometryExEx::VolumeRegularizer, py::arg("seed") = -1)
+        .def_readwrite("fused_edge_loss", &GeometryExEx::fused_edge_loss)
         .def("AccumulateSampleLossPerNode", &GeometryExEx::AccumulateSampleLossPerNode)
         .def_property_readonly("decoder", [](GeometryExEx& g) { return g.decoder.ptr(); })
         .def_property_readonly("grid", [](GeometryExEx& g) { return g.explicit_grid_generator->grid_data; });

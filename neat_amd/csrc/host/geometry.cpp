@@ -211,7 +211,7 @@ struct TVLossFn : public torch::autograd::Function<TVLossFn>
         const int64_t st[5] = {grid.stride(0), grid.stride(1), grid.stride(2), grid.stride(3), grid.stride(4)};
         auto loss = torch::empty({1}, grid.options().dtype(torch::kFloat64));
         const int64_t A = slots.size(0), C = grid.size(1), G = grid.size(2);
-        const size_t wsb = neat_reg_workspace_bytes(A * C * G * G * G);
+        const size_t wsb = neat_tv_workspace_bytes(A, (int32_t)C, (int32_t)G);
         auto ws = torch::empty({(int64_t)std::max<size_t>(wsb, 16)}, grid.options().dtype(torch::kUInt8));
         NEAT_CALL(neat_tv_loss_fwd(current_stream(), dptr<float>(grid), grid.size(0), (int32_t)C, (int32_t)G, st,
                                    dptr<int64_t>(slots), A, weight, dptr<double>(loss), ws.data_ptr(), wsb));
@@ -268,6 +268,57 @@ struct EdgeLossFn : public torch::autograd::Function<EdgeLossFn>
     }
 };
 
+// The same edge loss in one fused pass over the candidate face points (neat_edge_loss_direct): the neighbour rows,
+// their local coordinates and features are never materialised; backward recomputes the pairs (they depend on the
+// tree only) and adds the grid gradient with atomics. Values and gradients equal NodeNeighborSamples ->
+// IndirectGridSample3D -> EdgeLossFn up to the order of the FP64 loss sum and of the float atomics.
+struct FaceEdgeLossFn : public torch::autograd::Function<FaceEdgeLossFn>
+{
+    static void call(const torch::Tensor& grid, const std::vector<torch::Tensor>& t, int64_t seed, double weight,
+                     double* loss, const float* grad_loss, float* grad_grid)
+    {
+        const auto &pmin = t[0], &pmax = t[1], &flags = t[2], &ids = t[3], &slot = t[4];
+        const int64_t st[5] = {grid.stride(0), grid.stride(1), grid.stride(2), grid.stride(3), grid.stride(4)};
+        const int64_t A = ids.size(0);
+        const int32_t G = (int32_t)grid.size(2);
+        const size_t wsb = neat_edge_loss_direct_workspace_bytes(A, G);
+        auto ws = torch::empty({(int64_t)std::max<size_t>(wsb, 16)}, grid.options().dtype(torch::kUInt8));
+        NEAT_CALL(neat_edge_loss_direct(current_stream(), dptr<float>(pmin), dptr<float>(pmax),
+                                        flags.data_ptr<uint8_t>(), (int32_t)pmin.size(0), dptr<int64_t>(ids), A,
+                                        dptr<int64_t>(slot), dptr<float>(grid), grid.size(0), (int32_t)grid.size(1), G,
+                                        st, 0.001f, (uint64_t)seed, weight, loss, grad_loss, grad_grid, ws.data_ptr(),
+                                        wsb));
+    }
+
+    static torch::Tensor forward(torch::autograd::AutogradContext* ctx, torch::Tensor grid, torch::Tensor pos_min,
+                                 torch::Tensor pos_max, torch::Tensor flags, torch::Tensor active_ids,
+                                 torch::Tensor node_slot, int64_t seed, double weight)
+    {
+        TORCH_CHECK(grid.dim() == 5 && grid.size(2) == grid.size(3) && grid.size(3) == grid.size(4),
+                    "edge loss: a cubic grid [N, C, G, G, G] is expected");
+        auto loss = torch::empty({1}, grid.options().dtype(torch::kFloat64));
+        call(grid, {pos_min, pos_max, flags, active_ids, node_slot}, seed, weight, dptr<double>(loss), nullptr, nullptr);
+        ctx->save_for_backward({grid, pos_min, pos_max, flags, active_ids, node_slot});
+        ctx->saved_data["seed"]   = seed;
+        ctx->saved_data["weight"] = weight;
+        return loss.to(torch::kFloat32).reshape({});
+    }
+
+    static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
+                                                   torch::autograd::variable_list grad_out)
+    {
+        auto sv   = ctx->get_saved_variables();
+        auto grid = sv[0];
+        auto gl   = grad_out[0].to(torch::kFloat32).reshape({1}).contiguous();
+        auto gg   = torch::zeros_like(grid);
+        TORCH_CHECK(gg.strides() == grid.strides(), "edge loss: gradient layout");
+        call(grid, {sv[1], sv[2], sv[3], sv[4], sv[5]}, ctx->saved_data["seed"].toInt(),
+             ctx->saved_data["weight"].toDouble(), nullptr, dptr<float>(gl), dptr<float>(gg));
+        return {gg, torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor(),
+                torch::Tensor(), torch::Tensor()};
+    }
+};
+
 torch::Tensor GeometryExEx::VolumeRegularizer(int64_t seed)
 {
     auto& tp  = params->train_params;
@@ -280,17 +331,27 @@ torch::Tensor GeometryExEx::VolumeRegularizer(int64_t seed)
     }
     if (tp.loss_edge > 0)
     {
-        const int G = params->net_params.grid_size;
-        SampleList nb = tree->NodeNeighborSamples({G, G, G}, 0.001);
-        if (nb.size() > 0)
+        if (seed < 0) seed = torch::randint(0, (int64_t)1 << 62, {1}, torch::kLong).item<int64_t>();
+        torch::Tensor edge;
+        if (fused_edge_loss)
         {
-            auto local_samples   = tree->ComputeLocalSamples(nb.global_coordinate, nb.node_id);
-            auto local_node_id   = tree->GlobalNodeIdToLocalActiveId(nb.node_id);
-            auto neural_features = IndirectGridSample3D(grid, local_node_id, local_samples);
-            if (seed < 0) seed = torch::randint(0, (int64_t)1 << 62, {1}, torch::kLong).item<int64_t>();
-            auto edge = EdgeLossFn::apply(neural_features, seed, tp.loss_edge);
-            loss      = loss.defined() ? loss + edge : edge;
+            edge = FaceEdgeLossFn::apply(grid, tree->node_position_min, tree->node_position_max, tree->node_flags,
+                                         tree->active_node_ids.to(torch::kLong).contiguous(),
+                                         tree->node_active_prefix_sum, seed, tp.loss_edge);
         }
+        else
+        {
+            const int G   = params->net_params.grid_size;
+            SampleList nb = tree->NodeNeighborSamples({G, G, G}, 0.001);
+            if (nb.size() > 0)
+            {
+                auto local_samples   = tree->ComputeLocalSamples(nb.global_coordinate, nb.node_id);
+                auto local_node_id   = tree->GlobalNodeIdToLocalActiveId(nb.node_id);
+                auto neural_features = IndirectGridSample3D(grid, local_node_id, local_samples);
+                edge                 = EdgeLossFn::apply(neural_features, seed, tp.loss_edge);
+            }
+        }
+        if (edge.defined()) loss = loss.defined() ? loss + edge : edge;
     }
     return loss;
 }

@@ -91,6 +91,9 @@ class GeometryExEx : public HierarchicalNeuralGeometry
     // nodes times loss_edge (geometry_ex_ex.cpp:86-160); differentiable with respect to the grid. seed < 0 draws the
     // per-pair factors' seed from torch's default generator (the reference draws torch::rand_like).
     torch::Tensor VolumeRegularizer(int64_t seed = -1);
+    // true: the edge loss in one fused pass over the candidate face points; false: the reference's chain
+    // (NodeNeighborSamples, ComputeLocalSamples, IndirectGridSample3D, edge loss). Same values either way.
+    bool fused_edge_loss = true;
 
     ExplicitFeatureGrid explicit_grid_generator = nullptr;
 };

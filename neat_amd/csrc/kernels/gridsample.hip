@@ -5,65 +5,10 @@
 // run and are read with 16-byte vector loads. The arithmetic (corner weights from the unclamped neighbours, indices
 // clamped to the border, per-channel sum in the corner order tnw tne tsw tse bnw bne bsw bse) follows the reference
 // and is compiled without contraction, so the forward is bit-identical to the oracle.
-#include "common.hpp"
+#include "gridsample_dev.hpp"
 
 namespace neat
 {
-
-struct GridArgs
-{
-    const float* grid;
-    int64_t s0, s1, s2, s3, s4;  // element strides of (N, C, Z, Y, X)
-    int64_t N;
-    int32_t C, Z, Y, X;
-};
-
-struct Tri
-{
-    float w[8];
-    int64_t off[8];  // element offset of each clamped corner (without the channel term)
-    float ix, iy, iz;
-    int32_t x0, y0, z0;
-};
-
-__device__ __forceinline__ int clip_i(int v, int lim) { return min(lim - 1, max(v, 0)); }
-
-__device__ __forceinline__ Tri trilinear(const GridArgs& g, int64_t n, float u, float v, float wz)
-{
-    Tri t;
-    float ix = ((u + 1) / 2) * (float)(g.X - 1);
-    float iy = ((v + 1) / 2) * (float)(g.Y - 1);
-    float iz = ((wz + 1) / 2) * (float)(g.Z - 1);
-    int x0 = (int)floorf(ix), y0 = (int)floorf(iy), z0 = (int)floorf(iz);
-    int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
-    t.w[0] = ((float)x1 - ix) * ((float)y1 - iy) * ((float)z1 - iz);
-    t.w[1] = (ix - (float)x0) * ((float)y1 - iy) * ((float)z1 - iz);
-    t.w[2] = ((float)x1 - ix) * (iy - (float)y0) * ((float)z1 - iz);
-    t.w[3] = (ix - (float)x0) * (iy - (float)y0) * ((float)z1 - iz);
-    t.w[4] = ((float)x1 - ix) * ((float)y1 - iy) * (iz - (float)z0);
-    t.w[5] = (ix - (float)x0) * ((float)y1 - iy) * (iz - (float)z0);
-    t.w[6] = ((float)x1 - ix) * (iy - (float)y0) * (iz - (float)z0);
-    t.w[7] = (ix - (float)x0) * (iy - (float)y0) * (iz - (float)z0);
-    int cx0 = clip_i(x0, g.X), cx1 = clip_i(x1, g.X);
-    int cy0 = clip_i(y0, g.Y), cy1 = clip_i(y1, g.Y);
-    int cz0 = clip_i(z0, g.Z), cz1 = clip_i(z1, g.Z);
-    int64_t b = n * g.s0;
-    t.off[0]  = b + cz0 * g.s2 + cy0 * g.s3 + cx0 * g.s4;
-    t.off[1]  = b + cz0 * g.s2 + cy0 * g.s3 + cx1 * g.s4;
-    t.off[2]  = b + cz0 * g.s2 + cy1 * g.s3 + cx0 * g.s4;
-    t.off[3]  = b + cz0 * g.s2 + cy1 * g.s3 + cx1 * g.s4;
-    t.off[4]  = b + cz1 * g.s2 + cy0 * g.s3 + cx0 * g.s4;
-    t.off[5]  = b + cz1 * g.s2 + cy0 * g.s3 + cx1 * g.s4;
-    t.off[6]  = b + cz1 * g.s2 + cy1 * g.s3 + cx0 * g.s4;
-    t.off[7]  = b + cz1 * g.s2 + cy1 * g.s3 + cx1 * g.s4;
-    t.ix      = ix;
-    t.iy      = iy;
-    t.iz      = iz;
-    t.x0      = x0;
-    t.y0      = y0;
-    t.z0      = z0;
-    return t;
-}
 
 // Channel-last fast path: C4 groups of 4 channels read with one 16-byte load per corner.
 template <bool kChannelsLast>

@@ -364,7 +364,7 @@ def tv_loss_fwd(grid, slots, weight, stream=None):
     st = _strides(grid)
     A = int(slots.shape[0])
     out = torch.empty(1, dtype=torch.float64, device=grid.device)
-    ws = workspace(_abi.lib().neat_reg_workspace_bytes(A * C * G * G * G), grid.device)
+    ws = workspace(_abi.lib().neat_tv_workspace_bytes(A, C, G), grid.device)
     call("neat_tv_loss_fwd", stream_ptr(stream), ptr(grid), N, C, G, st.ctypes.data, ptr(slots.contiguous()), A,
          float(weight), ptr(out), ptr(ws), ws.numel())
     return out
@@ -379,10 +379,67 @@ def tv_loss_bwd(grid, slots, weight, grad_loss, grad_grid, stream=None):
          int(slots.shape[0]), float(weight), ptr(grad_loss.to(_F32).contiguous()), ptr(grad_grid))
 
 
+def edge_loss_direct(pos_min, pos_max, flags, prefix, active_ids, grid, seed, weight, epsilon=0.001, want_loss=True,
+                     grad_loss=None, grad_grid=None, stream=None):
+    """The edge loss in one pass over the candidate face points (neat_edge_loss_direct): returns the FP64 loss tensor
+    (or None) and adds d loss / d grid * grad_loss to grad_grid when given (same strides as grid)."""
+    N, C, G = grid.shape[0], grid.shape[1], grid.shape[2]
+    A = int(active_ids.shape[0])
+    st = _strides(grid)
+    loss = torch.empty(1, dtype=torch.float64, device=grid.device) if want_loss else None
+    if grad_grid is not None:
+        assert grad_grid.stride() == grid.stride()
+        grad_loss = grad_loss.to(_F32).contiguous()
+    ws = workspace(_abi.lib().neat_edge_loss_direct_workspace_bytes(A, G), grid.device)
+    call("neat_edge_loss_direct", stream_ptr(stream), ptr(pos_min), ptr(pos_max), ptr(flags), pos_min.shape[0],
+         ptr(active_ids.contiguous()), A, ptr(prefix), ptr(grid), N, C, G, st.ctypes.data, float(epsilon), int(seed),
+         float(weight), ptr(loss), ptr(grad_loss) if grad_grid is not None else None, ptr(grad_grid), ptr(ws),
+         ws.numel())
+    return loss
+
+
+def tv_loss_fwd_bwd(grid, slots, weight, grad_loss, grad_grid, overwrite=False, stream=None):
+    """Total variation and its gradient in one pass: returns the FP64 loss; writes (overwrite) or adds
+    d tv / d grid * grad_loss into grad_grid's given slots."""
+    N, C, G = grid.shape[0], grid.shape[1], grid.shape[2]
+    assert grad_grid.stride() == grid.stride()
+    st = _strides(grid)
+    A = int(slots.shape[0])
+    out = torch.empty(1, dtype=torch.float64, device=grid.device)
+    ws = workspace(_abi.lib().neat_tv_workspace_bytes(A, C, G), grid.device)
+    call("neat_tv_loss_fwd_bwd", stream_ptr(stream), ptr(grid), N, C, G, st.ctypes.data, ptr(slots.contiguous()), A,
+         float(weight), ptr(out), ptr(grad_loss.to(_F32).contiguous()), ptr(grad_grid), int(bool(overwrite)), ptr(ws),
+         ws.numel())
+    return out
+
+
 def volume_regularizer(pos_min, pos_max, flags, prefix, active_ids, grid, loss_tv=1e-4, loss_edge=1e-3, seed=0,
                        epsilon=0.001, stream=None):
     """GeometryExEx::VolumeRegularizer's value and grid gradient: tv (loss_tv > 0) plus the edge loss (loss_edge > 0)
-    along the faces of neighbouring active nodes. Returns (loss FP64 scalar tensor, grad_grid)."""
+    along the faces of neighbouring active nodes, the edge loss in one fused pass. Returns (loss FP64 scalar tensor,
+    grad_grid)."""
+    one = torch.ones(1, dtype=_F32, device=grid.device)
+    loss = torch.zeros(1, dtype=torch.float64, device=grid.device)
+    if loss_tv > 0:
+        # the active nodes hold slots 0 .. A-1 (prefix is the exclusive prefix sum of the active flags), which the
+        # total variation's gradient overwrites; the remaining slots are cleared
+        A = int(active_ids.shape[0])
+        gg = torch.empty_like(grid)
+        if grid.shape[0] > A:
+            gg[A:].zero_()
+        loss += tv_loss_fwd_bwd(grid, prefix[active_ids], loss_tv, one, gg, overwrite=True, stream=stream)
+    else:
+        gg = torch.zeros_like(grid)
+    if loss_edge > 0:
+        loss += edge_loss_direct(pos_min, pos_max, flags, prefix, active_ids, grid, seed, loss_edge, epsilon,
+                                 grad_loss=one, grad_grid=gg, stream=stream)
+    return loss, gg
+
+
+def volume_regularizer_unfused(pos_min, pos_max, flags, prefix, active_ids, grid, loss_tv=1e-4, loss_edge=1e-3,
+                               seed=0, epsilon=0.001, stream=None):
+    """The same value and gradient through the reference's chain of steps (neighbour rows, local coordinates, grid
+    sample, edge loss, and back through the grid sample)."""
     G = grid.shape[2]
     gg = torch.zeros_like(grid)
     one = torch.ones(1, dtype=_F32, device=grid.device)

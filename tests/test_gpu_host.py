@@ -177,13 +177,16 @@ def test_compute_image_precision_forms(H, oracle, precision):
         assert max(e) <= 1e-5
 
 
-def test_volume_regularizer_module_vs_oracle(H, oracle):
+@pytest.mark.parametrize("fused", [True, False])
+def test_volume_regularizer_module_vs_oracle(H, oracle, fused):
     """GeometryExEx::VolumeRegularizer (geometry_ex_ex.cpp:86-160): loss and grid gradient through autograd against
-    the oracle's composition (neighbour pairs, local coordinates, grid sample, edge loss; total variation)."""
+    the oracle's composition (neighbour pairs, local coordinates, grid sample, edge loss; total variation), with the
+    edge loss fused and as the reference's chain of steps."""
     from oracle import regularizer as R
-    print("case: GeometryExEx.VolumeRegularizer C1", flush=True)
+    print("case: GeometryExEx.VolumeRegularizer C1 fused", fused, flush=True)
     cfg = W.C1
     sc, _, _, geo = make_geometry(H, cfg, seed=5)
+    geo.fused_edge_loss = fused
     p = geo.grid
     tree = sc["tree"]
     ids, prefix = sc["active_ids"], sc["prefix"]
