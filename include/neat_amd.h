@@ -381,6 +381,21 @@ int neat_edge_loss_direct(neat_stream_t stream, const float* node_pos_min, const
                           int32_t G, const int64_t* strides, float epsilon, uint64_t seed, double weight, double* loss,
                           const float* grad_loss, float* grad_grid, void* workspace, size_t workspace_bytes);
 
+/* ---- camera poses of the ray generation (FP64) ----------------------------------------------------------------------------
+ * Quaternions (x, y, z, w) per pose, [num_poses, 4]; points [num_points, 3]; index[i] = pose of point i (int32, as
+ * the reference's image ids).
+ * neat_rotate_points_fwd: out = q[index] * p (RotatePointGPUk, CameraPose.cu:20-34).
+ * neat_rotate_points_bwd: grad_points = R^T grad_out (nullable); grad_tangent[pose] += (R p) x grad_out, the gradient
+ * with respect to a left tangent perturbation exp(t) * q at t = 0, accumulated with FP64 atomics (nullable;
+ * RotatePointBackwardGPUk, CameraPose.cu:37-63).
+ * neat_so3_apply_tangent: q <- exp(t) * q (renormalised), t <- 0 (ApplyTangent, CameraPose.cu:160-180). */
+int neat_rotate_points_fwd(neat_stream_t stream, const double* points, const int32_t* index, const double* rotation,
+                           int64_t num_points, int64_t num_poses, double* out);
+int neat_rotate_points_bwd(neat_stream_t stream, const double* points, const int32_t* index, const double* rotation,
+                           int64_t num_points, int64_t num_poses, const double* grad_out, double* grad_tangent,
+                           double* grad_points);
+int neat_so3_apply_tangent(neat_stream_t stream, double* tangent, double* rotation, int64_t num_poses);
+
 /* ---- per-node loss of the evaluation pass --------------------------------------------------------------------------------
  * HierarchicalNeuralGeometry::AccumulateSampleLossPerNode (geometry.cpp:38-58): loss_sum[node] = sum over the node's
  * samples of per_ray_loss[ray] * weight, weight_sum[node] = sum of the weights, float [num_nodes], over the canonical

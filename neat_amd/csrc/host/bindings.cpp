@@ -3,6 +3,7 @@
 #include <torch/extension.h>
 #include <torch/python.h>
 
+#include "CameraPose.h"
 #include "geometry.h"
 
 namespace py = pybind11;
@@ -119,6 +120,31 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     module_methods(grid_cls);
     grid_cls.def(py::init<int, int, std::vector<long>, std::string>())
         .def_readonly("grid_data", &ExplicitFeatureGridImpl::grid_data);
+
+    m.def("UVToPixel", &UVToPixel, py::arg("uv"), py::arg("w"), py::arg("h"), py::arg("align_corners"));
+    m.def("PixelToUV", &PixelToUV, py::arg("px"), py::arg("w"), py::arg("h"), py::arg("align_corners"));
+    py::class_<CameraPoseModuleImpl, std::shared_ptr<CameraPoseModuleImpl>> pose_cls(m, "CameraPoseModule");
+    module_methods(pose_cls);
+    pose_cls.def(py::init<torch::Tensor, torch::Tensor>(), py::arg("rotation"), py::arg("translation"))
+        .def("RotatePoint", &CameraPoseModuleImpl::RotatePoint)
+        .def("ApplyTangent", &CameraPoseModuleImpl::ApplyTangent)
+        .def_readonly("rotation", &CameraPoseModuleImpl::rotation)
+        .def_readonly("rotation_tangent", &CameraPoseModuleImpl::rotation_tangent)
+        .def_readonly("translation", &CameraPoseModuleImpl::translation);
+    py::class_<CameraModelModuleImpl, std::shared_ptr<CameraModelModuleImpl>> cam_cls(m, "CameraModelModule");
+    module_methods(cam_cls);
+    cam_cls.def(py::init<int, int, torch::Tensor>(), py::arg("h"), py::arg("w"), py::arg("intrinsics"))
+        .def("Unproject", &CameraModelModuleImpl::Unproject)
+        .def_readonly("intrinsics", &CameraModelModuleImpl::intrinsics)
+        .def_readonly("h", &CameraModelModuleImpl::h)
+        .def_readonly("w", &CameraModelModuleImpl::w);
+    m.def(
+        "GetRays",
+        [](std::shared_ptr<CameraPoseModuleImpl> pose, std::shared_ptr<CameraModelModuleImpl> cam, torch::Tensor uv,
+           torch::Tensor image_id, torch::Tensor camera_id, bool align)
+        { return GetRays(CameraPoseModule(pose), CameraModelModule(cam), uv, image_id, camera_id, align); },
+        py::arg("pose"), py::arg("camera"), py::arg("uv"), py::arg("image_id"), py::arg("camera_id"),
+        py::arg("uv_align_corners") = false);
 
     py::class_<Netparams>(m, "Netparams")
         .def(py::init<>())

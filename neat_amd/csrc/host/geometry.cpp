@@ -181,7 +181,11 @@ torch::Tensor HierarchicalNeuralGeometry::DecodeFeatures(torch::Tensor neural_fe
 
 torch::Tensor HierarchicalNeuralGeometry::ComputeImage(SampleList all_samples, int num_channels, int num_pixels)
 {
-    if (params->train_params.use_fused_projector && all_samples.segments.defined() && num_channels == 1)
+    // the fused projector returns no positional gradient: samples whose coordinates carry one (rays that require
+    // grad, e.g. from GetRays with pose refinement) take the kernel chain, whose grid sample backward yields d uv
+    const bool positional_grad = all_samples.global_coordinate.defined() && all_samples.global_coordinate.requires_grad();
+    if (params->train_params.use_fused_projector && all_samples.segments.defined() && num_channels == 1 &&
+        !positional_grad)
     {
         auto img = ComputeImageFused(all_samples, num_pixels);
         if (img.defined()) return img;

@@ -476,3 +476,31 @@ def node_loss_accumulate(segs, per_ray_loss, num_nodes, jitter=False, seed=0, st
          int(bool(jitter)), int(seed), ptr(per_ray_loss.to(_F32).contiguous()), num_nodes, ptr(ls), ptr(wsum),
          ptr(ws), ws.numel())
     return ls, wsum
+
+
+# ---- camera poses of the ray generation ---------------------------------------------------------------------------------
+def rotate_points(rotation, points, index, stream=None):
+    """q[index] * p in FP64 (neat_rotate_points_fwd)."""
+    p = points.to(torch.float64).contiguous()
+    idx = index.to(torch.int32).contiguous()
+    out = torch.empty_like(p)
+    call("neat_rotate_points_fwd", stream_ptr(stream), ptr(p), ptr(idx), ptr(rotation.contiguous()), p.shape[0],
+         rotation.shape[0], ptr(out))
+    return out
+
+
+def rotate_points_bwd(rotation, points, index, grad_out, stream=None):
+    """(grad_tangent [poses, 3], grad_points [N, 3]) of rotate_points (neat_rotate_points_bwd)."""
+    p = points.to(torch.float64).contiguous()
+    idx = index.to(torch.int32).contiguous()
+    gt = torch.zeros(rotation.shape[0], 3, dtype=torch.float64, device=p.device)
+    gp = torch.empty_like(p)
+    call("neat_rotate_points_bwd", stream_ptr(stream), ptr(p), ptr(idx), ptr(rotation.contiguous()), p.shape[0],
+         rotation.shape[0], ptr(grad_out.to(torch.float64).contiguous()), ptr(gt), ptr(gp))
+    return gt, gp
+
+
+def so3_apply_tangent(tangent, rotation, stream=None):
+    """rotation <- exp(tangent) * rotation, tangent <- 0, in place (neat_so3_apply_tangent)."""
+    assert tangent.is_contiguous() and rotation.is_contiguous()
+    call("neat_so3_apply_tangent", stream_ptr(stream), ptr(tangent), ptr(rotation), rotation.shape[0])
