@@ -396,6 +396,16 @@ int neat_rotate_points_bwd(neat_stream_t stream, const double* points, const int
                            double* grad_points);
 int neat_so3_apply_tangent(neat_stream_t stream, double* tangent, double* rotation, int64_t num_poses);
 
+/* ---- uniform volume samples of the volume export --------------------------------------------------------------------------
+ * HyperTreeBaseImpl::UniformPhantomSamplesGPU (HyperTree.cu:547-597) with NodeIdForPositionGPU (:643-686): voxel i of
+ * a size[0] x size[1] x size[2] lattice (dimension 0 fastest; the last with swap_xy) at c / (size - 1) * 2 - 1;
+ * out_coord float [n, 3], out_index int64 [n] (= i), out_node int64 [n] (the active node containing it, else
+ * first_active_node), out_weight float [n] (1, else 0); size is a host array of 3. */
+int neat_uniform_volume_samples(neat_stream_t stream, const float* node_pos_min, const float* node_pos_max,
+                                const uint8_t* node_flags, int32_t num_nodes, int64_t first_active_node,
+                                const int32_t* size, int32_t swap_xy, float* out_coord, int64_t* out_index,
+                                int64_t* out_node, float* out_weight);
+
 /* ---- per-node loss of the evaluation pass --------------------------------------------------------------------------------
  * HierarchicalNeuralGeometry::AccumulateSampleLossPerNode (geometry.cpp:38-58): loss_sum[node] = sum over the node's
  * samples of per_ray_loss[ray] * weight, weight_sum[node] = sum of the weights, float [num_nodes], over the canonical

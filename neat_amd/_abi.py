@@ -96,6 +96,7 @@ SIGNATURES.update({
     "neat_rotate_points_fwd": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I64, _P]),
     "neat_rotate_points_bwd": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I64, _P, _P, _P]),
     "neat_so3_apply_tangent": (ctypes.c_int, [_P, _P, _P, _I64]),
+    "neat_uniform_volume_samples": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I64, _P, _I32, _P, _P, _P, _P]),
     "neat_tv_loss_fwd_bwd": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P, _I64, ctypes.c_double, _P, _P, _P,
                                             _I32, _P, _SZ]),
     "neat_edge_loss_fwd": (ctypes.c_int, [_P, _P, _I64, _I32, _U64, ctypes.c_double, _P, _P, _SZ]),

@@ -267,6 +267,30 @@ SampleList HyperTreeBaseImpl::NodeNeighborSamples(const std::vector<int>& size, 
     return list;
 }
 
+SampleList HyperTreeBaseImpl::UniformPhantomSamplesGPU(const std::vector<int>& size, bool swap_xy)
+{
+    check_cuda(node_position_min, "the octree");
+    TORCH_CHECK(size.size() == 3, "UniformPhantomSamplesGPU: a 3-d size is expected");
+    TORCH_CHECK(NumActiveNodes() > 0, "UniformPhantomSamplesGPU: the tree has no active node");
+    const auto dev  = node_position_min.device();
+    const int64_t n = (int64_t)size[0] * size[1] * size[2];
+    SampleList list;
+    list.global_coordinate  = torch::empty({n, 3}, torch::TensorOptions().dtype(torch::kFloat32).device(dev));
+    list.ray_index          = torch::empty({n}, torch::TensorOptions().dtype(torch::kLong).device(dev));
+    list.node_id            = torch::empty({n}, torch::TensorOptions().dtype(torch::kLong).device(dev));
+    list.weight             = torch::empty({n}, torch::TensorOptions().dtype(torch::kFloat32).device(dev));
+    list.local_index_in_ray = torch::zeros({n}, torch::TensorOptions().dtype(torch::kLong).device(dev));
+    list.ray_t              = torch::zeros({n}, torch::TensorOptions().dtype(torch::kFloat32).device(dev));
+    const int32_t sz[3]     = {size[0], size[1], size[2]};
+    const int64_t first     = active_node_ids.to(torch::kLong).cpu()[0].item<int64_t>();
+    NEAT_CALL(neat_uniform_volume_samples(current_stream(), dptr<float>(node_position_min),
+                                          dptr<float>(node_position_max), node_flags.data_ptr<uint8_t>(), NumNodes(),
+                                          first, sz, swap_xy ? 1 : 0, dptr<float>(list.global_coordinate),
+                                          dptr<int64_t>(list.ray_index), dptr<int64_t>(list.node_id),
+                                          dptr<float>(list.weight)));
+    return list;
+}
+
 void HyperTreeBaseImpl::SetErrorForActiveNodes(torch::Tensor error, std::string strategy)
 {
     // elementwise over [num_nodes], the reference's tensor expressions (control-plane bookkeeping, not a hot path)

@@ -30,6 +30,8 @@ class HyperTreeBaseImpl : public torch::nn::Module
     // in canonical order (active node, side, front, face point). `size` holds the per-axis face resolution (the
     // grid size; cubic only), epsilon the offset that finds the neighbour.
     SampleList NodeNeighborSamples(const std::vector<int>& size, double epsilon);
+    // voxels of a size[0] x size[1] x size[2] lattice over [-1, 1]^3 with their active node (HyperTree.cu:547-597)
+    SampleList UniformPhantomSamplesGPU(const std::vector<int>& size, bool swap_xy);
 
     // u = (g - min)/(max - min) * 2 - 1 for float [N, D] or [groups, group_size, D] samples, differentiable with
     // respect to global_samples (HyperTree.h:134, HyperTree.cu:116-199).

@@ -81,6 +81,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("CreateSegmentsForRays", &HyperTreeBaseImpl::CreateSegmentsForRays)
         .def("ComputeLocalSamples", &HyperTreeBaseImpl::ComputeLocalSamples)
         .def("GlobalNodeIdToLocalActiveId", &HyperTreeBaseImpl::GlobalNodeIdToLocalActiveId)
+        .def("UniformPhantomSamplesGPU", &HyperTreeBaseImpl::UniformPhantomSamplesGPU, py::arg("size"),
+             py::arg("swap_xy") = false)
         .def("NodeNeighborSamples", &HyperTreeBaseImpl::NodeNeighborSamples)
         .def("SetErrorForActiveNodes", &HyperTreeBaseImpl::SetErrorForActiveNodes, py::arg("error"),
              py::arg("strategy") = "override")
@@ -195,6 +197,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("VolumeRegularizer", &GeometryExEx::VolumeRegularizer, py::arg("seed") = -1)
         .def_readwrite("fused_edge_loss", &GeometryExEx::fused_edge_loss)
         .def("AccumulateSampleLossPerNode", &GeometryExEx::AccumulateSampleLossPerNode)
+        .def("UniformSampledVolume", &GeometryExEx::UniformSampledVolume, py::arg("shape"), py::arg("num_channels"))
         .def_property_readonly("decoder", [](GeometryExEx& g) { return g.decoder.ptr(); })
         .def_property_readonly("grid", [](GeometryExEx& g) { return g.explicit_grid_generator->grid_data; });
 }

@@ -360,6 +360,31 @@ torch::Tensor GeometryExEx::VolumeRegularizer(int64_t seed)
     return loss;
 }
 
+std::tuple<torch::Tensor, torch::Tensor, torch::Tensor> HierarchicalNeuralGeometry::UniformSampledVolume(
+    std::vector<long> shape, int num_channels)
+{
+    TORCH_CHECK(shape.size() == 3, "UniformSampledVolume: a 3-d shape is expected");
+    torch::NoGradGuard ng;
+    auto all      = tree->UniformPhantomSamplesGPU({(int)shape[0], (int)shape[1], (int)shape[2]}, false);
+    const int64_t n = all.global_coordinate.size(0);
+    auto values   = torch::zeros({num_channels, n}, all.weight.options());
+    const int64_t batch = (int64_t)1 << 20;
+    for (int64_t b = 0; b < n; b += batch)
+    {
+        const int64_t e = std::min(n, b + batch);
+        auto out = SampleVolumeIndirect(all.global_coordinate.slice(0, b, e), all.node_id.slice(0, b, e));
+        TORCH_CHECK(out.defined() && out.size(1) == num_channels, "UniformSampledVolume: the geometry gives ",
+                    out.defined() ? out.size(1) : 0, " channels, ", num_channels, " requested");
+        // model output times the integration weight (0 outside the active nodes)
+        values.slice(1, b, e).copy_((out * all.weight.slice(0, b, e).unsqueeze(1)).t());
+    }
+    auto valid   = all.weight.to(torch::kLong);
+    auto node_id = all.node_id * valid;
+    std::vector<long> vshape = shape;
+    vshape.insert(vshape.begin(), num_channels);
+    return {values.reshape(vshape).cpu(), node_id.reshape(shape).cpu(), valid.reshape(shape).cpu()};
+}
+
 std::pair<torch::Tensor, torch::Tensor> HierarchicalNeuralGeometry::AccumulateSampleLossPerNode(
     const SampleList& combined_samples, torch::Tensor per_ray_loss)
 {

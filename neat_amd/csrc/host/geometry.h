@@ -56,6 +56,11 @@ class HierarchicalNeuralGeometry : public NeuralGeometry
     // Per-node sums of the samples' per-ray loss times weight, and of the weights (geometry.cpp:38-58), float
     // [num_nodes] each; per_ray_loss is [channels, num_rays] (averaged over channels). Needs the samples' segment
     // list (CreateSamplesForRays keeps it in SampleList::segments).
+    // the field on a uniform lattice over [-1, 1]^3 (geometry.cpp:166-249): volume [num_channels, shape...] (the
+    // decoded value where the voxel lies in an active node, 0 elsewhere), node id and validity [shape...] (int64),
+    // on the CPU as there; evaluated in batches of 2^20 voxels on the GPU
+    std::tuple<torch::Tensor, torch::Tensor, torch::Tensor> UniformSampledVolume(std::vector<long> shape,
+                                                                                 int num_channels);
     std::pair<torch::Tensor, torch::Tensor> AccumulateSampleLossPerNode(const SampleList& combined_samples,
                                                                         torch::Tensor per_ray_loss);
 
