@@ -123,6 +123,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     grid_cls.def(py::init<int, int, std::vector<long>, std::string>())
         .def_readonly("grid_data", &ExplicitFeatureGridImpl::grid_data);
 
+    m.def("SaveCheckpoint", &SaveCheckpoint, py::arg("geometry"), py::arg("tree"), py::arg("prefix"));
+    m.def("LoadCheckpoint", &LoadCheckpoint, py::arg("geometry"), py::arg("tree"), py::arg("prefix"));
     m.def("UVToPixel", &UVToPixel, py::arg("uv"), py::arg("w"), py::arg("h"), py::arg("align_corners"));
     m.def("PixelToUV", &PixelToUV, py::arg("px"), py::arg("w"), py::arg("h"), py::arg("align_corners"));
     py::class_<CameraPoseModuleImpl, std::shared_ptr<CameraPoseModuleImpl>> pose_cls(m, "CameraPoseModule");

@@ -102,3 +102,10 @@ class GeometryExEx : public HierarchicalNeuralGeometry
 
     ExplicitFeatureGrid explicit_grid_generator = nullptr;
 };
+
+// torch::save / torch::load of the geometry and the tree as prefix + "geometry.pth" / "tree.pth"
+// (reconstruct.cpp:36-62)
+void SaveCheckpoint(std::shared_ptr<GeometryExEx> geometry, std::shared_ptr<HyperTreeBaseImpl> tree,
+                    const std::string& prefix);
+void LoadCheckpoint(std::shared_ptr<GeometryExEx> geometry, std::shared_ptr<HyperTreeBaseImpl> tree,
+                    const std::string& prefix);
