@@ -316,10 +316,10 @@ __global__ void __launch_bounds__(kRedThreads) k_sum_partials_per_pair(const dou
     if (threadIdx.x == 0) *out = P > 0 ? t * weight / (double)P : 0.0;
 }
 
-// Total variation, one thread per grid entry (kTvPerThread entries for the forward, strided by the block size), the
+// Total variation, kTvPerThread grid entries per thread (strided by the block size), the
 // entries of a node numbered in memory order (channel fastest when the grid is channels-last, x fastest otherwise) so
 // the entry and its three forward neighbours are coalesced loads. Blocks are laid out node-major (blocks_per_node per
-// node), so the within-node entry index stays 32-bit and is split with a float reciprocal and one correction step
+// node), so the within-node entry index stays 32-bit and is split by multiply-shift division (divmod_small)
 // (exact for indices below 2^24 and divisors below 256, which the host checks). When the node's entries are dense in
 // memory order (channels-last or plain contiguous), an entry's offset is its index and only the boundary tests need
 // the coordinates.
@@ -421,40 +421,14 @@ __global__ void __launch_bounds__(kRedThreads) k_tv_fwd(const float* __restrict_
     if (threadIdx.x == 0) partials[L] = tot;
 }
 
-// gradient of the total variation, accumulated into grad_grid, one entry per thread: entry v gets
-// sign(v - prev) - sign(next - v) per axis, times (*grad_loss) * scale, scale = weight * C / (A * C * G * G * (G - 1))
-__global__ void __launch_bounds__(256) k_tv_bwd(const float* __restrict__ grid, TvDims D, Strides5 S,
-                                                const int64_t* __restrict__ slots, double scale,
-                                                const float* __restrict__ grad_loss, float* __restrict__ grad_grid)
-{
-    const int64_t* st = S.s;
-    const uint32_t L = xcd_block(blockIdx.x, gridDim.x);
-    if (L >= D.blocks) return;  // a padding block
-    const int64_t a = L / D.blocks_per_node;
-    const int32_t e = (int32_t)(L - a * D.blocks_per_node) * 256 + threadIdx.x;
-    if (e >= D.per_node) return;
-    const int32_t G = D.G;
-    const TvEntry t = tv_entry(e, D);
-    const int64_t b = slots[a] * st[0] + (D.dense ? (int64_t)e : t.c * st[1] + t.z * st[2] + t.y * st[3] + t.x * st[4]);
-    const float* g0 = grid + b;
-    const float v0  = g0[0];
-    float s         = 0.f;
-    if (t.x > 0) s += sgn(v0 - g0[-st[4]]);
-    if (t.x + 1 < G) s -= sgn(g0[st[4]] - v0);
-    if (t.y > 0) s += sgn(v0 - g0[-st[3]]);
-    if (t.y + 1 < G) s -= sgn(g0[st[3]] - v0);
-    if (t.z > 0) s += sgn(v0 - g0[-st[2]]);
-    if (t.z + 1 < G) s -= sgn(g0[st[2]] - v0);
-    grad_grid[b] += (float)((double)*grad_loss * scale * s);
-}
-
 // Forward and backward of the total variation in one pass (kTvPerThread entries per thread, as the forward): the
-// loss partials as k_tv_fwd's, and the gradient of each entry written (overwrite) or added (accumulate) as k_tv_bwd's,
+// loss partials as k_tv_fwd's (kLoss), and the gradient of each entry, sign(v - prev) - sign(next - v) per axis times
+// (*grad_loss) * scale, written (overwrite) or added (accumulate); without kLoss it is the backward alone,
 // so the grid is read once and, when overwriting, the gradient buffer is neither cleared nor read. All loads of a
 // thread's entries are issued before any arithmetic, so a wave keeps 7 * kTvPerThread loads in flight; a missing
 // neighbour (at the grid's border) reads the entry itself, whose difference 0 adds nothing to either sum. kDense: the
 // entry's offset within its slot is its index (32-bit offsets).
-template <bool kOverwrite, bool kDense>
+template <bool kOverwrite, bool kDense, bool kLoss = true>
 __global__ void __launch_bounds__(kRedThreads) k_tv_fwd_bwd(const float* __restrict__ grid, TvDims D, Strides5 S,
                                                             const int64_t* __restrict__ slots, double scale,
                                                             const float* __restrict__ grad_loss,
@@ -465,7 +439,7 @@ __global__ void __launch_bounds__(kRedThreads) k_tv_fwd_bwd(const float* __restr
     const uint32_t L  = xcd_block(blockIdx.x, gridDim.x);
     if (L >= D.blocks)
     {
-        if (threadIdx.x == 0) partials[L] = 0.0;  // a padding block
+        if (kLoss && threadIdx.x == 0) partials[L] = 0.0;  // a padding block
         return;
     }
     const int64_t a  = L / D.blocks_per_node;
@@ -516,8 +490,11 @@ __global__ void __launch_bounds__(kRedThreads) k_tv_fwd_bwd(const float* __restr
         else
             d0[off[k]] += gv;
     }
-    const double tot = block_total(sum);
-    if (threadIdx.x == 0) partials[L] = tot;
+    if constexpr (kLoss)
+    {
+        const double tot = block_total(sum);
+        if (threadIdx.x == 0) partials[L] = tot;
+    }
 }
 
 }  // namespace neat
@@ -709,10 +686,15 @@ extern "C"
         NEAT_REQUIRE(grid && slots && grad_loss && grad_grid, "neat_tv_loss_bwd: null pointer");
         NEAT_REQUIRE(C * (int64_t)G * G * G < (1 << 24) && C < 256 && G < 256,
                      "neat_tv_loss_bwd: C * G^3 must be below 2^24, C and G below 256");
-        const TvDims D     = tv_dims(C, G, strides, 256, num_active);
+        const TvDims D     = tv_dims(C, G, strides, kRedThreads * kTvPerThread, num_active);
         const double scale = weight * C / ((double)num_active * C * G * G * (G - 1));
-        k_tv_bwd<<<(unsigned)xcd_grid(D.blocks), 256, 0, as_stream(stream)>>>(
-            grid, D, strides5(strides), slots, scale, grad_loss, grad_grid);
+        // the combined pass without its loss: same per-entry gradient, all of a thread's loads issued first
+        if (D.dense)
+            k_tv_fwd_bwd<false, true, false><<<(unsigned)xcd_grid(D.blocks), kRedThreads, 0, as_stream(stream)>>>(
+                grid, D, strides5(strides), slots, scale, grad_loss, grad_grid, nullptr);
+        else
+            k_tv_fwd_bwd<false, false, false><<<(unsigned)xcd_grid(D.blocks), kRedThreads, 0, as_stream(stream)>>>(
+                grid, D, strides5(strides), slots, scale, grad_loss, grad_grid, nullptr);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }
