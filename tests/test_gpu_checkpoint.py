@@ -44,3 +44,28 @@ def test_checkpoint_round_trip(H, tmp_path):
         i1 = geo.ComputeImage(tree.CreateSamplesForRays(rays, cfg.max_spn, False), 1, 2048)
         i2 = geo2.ComputeImage(tree2.CreateSamplesForRays(rays, cfg.max_spn, False), 1, 2048)
     assert torch.equal(i1, i2)
+
+
+def test_checkpoint_with_int32_active_prefix_loads(H, tmp_path):
+    """The reference registers node_active_id as int32 (HyperTree.cpp:41-53): a tree file holding it that way loads, and
+    the restored tree's derived state is rebuilt in the dtypes the kernels use."""
+    from test_gpu_host import make_geometry
+    print("case: checkpoint with an int32 node_active_id", flush=True)
+    cfg = W.C1
+    sc, tree, _, geo = make_geometry(H, cfg, seed=1)
+    want = {k: v.clone() for k, v in tree.named_buffers()}
+    buf = dict(tree.named_buffers())["node_active_id"]
+    buf.data = buf.data.to(torch.int32)
+    prefix = str(tmp_path / "ref_")
+    H.SaveCheckpoint(geo, tree, prefix)
+    sc2, tree2, _, geo2 = make_geometry(H, cfg, seed=9)
+    H.LoadCheckpoint(geo2, tree2, prefix)
+    got = dict(tree2.named_buffers())
+    assert got["node_active_id"].dtype == torch.int64
+    for k, v in want.items():
+        assert torch.equal(got[k].cpu(), v.cpu()), k
+    o, d = W.batch_rays(cfg, 1024, seed=4)
+    rays = H.RayList(torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV))
+    with torch.no_grad():
+        i1 = geo2.ComputeImage(tree2.CreateSamplesForRays(rays, cfg.max_spn, False), 1, 1024)
+    assert bool(torch.isfinite(i1).all())
