@@ -51,6 +51,9 @@ typedef void* neat_stream_t;
  * failing call on this thread. */
 const char* neat_last_error(void);
 int neat_abi_version(void);
+/* No reference counterpart. Hash of the sources this library was built from (16 hex digits; neat_amd/buildinfo.py
+ * computes it the same way), so a loader can refuse a library built from other sources. */
+const char* neat_build_id(void);
 
 /* ---- octree (host helpers, no device work) ------------------------------------------------------------------ */
 

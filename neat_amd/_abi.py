@@ -24,6 +24,7 @@ _SZ = ctypes.c_size_t
 SIGNATURES = {
     "neat_last_error": (ctypes.c_char_p, []),
     "neat_abi_version": (ctypes.c_int, []),
+    "neat_build_id": (ctypes.c_char_p, []),
     "neat_tree_num_nodes": (_I64, [_I32]),
     "neat_tree_build_host": (ctypes.c_int, [_I32, _P, _P, _P, _P, _P, _P]),
     "neat_tree_update_active_host": (_I64, [_P, _I32, _P, _P]),
@@ -130,8 +131,18 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+        from .buildinfo import source_id
+        built, want = L.neat_build_id().decode(), source_id("abi")
+        if built != want:
+            raise ImportError(f"{LIB_PATH} was built from other sources (build id {built}, sources {want}); rebuild "
+                              "with `python -m neat_amd.build`")
         _LIB = L
     return _LIB
+
+
+def build_id():
+    """Source hash compiled into the loaded library (equal to the sources' by construction of lib())."""
+    return lib().neat_build_id().decode()
 
 
 def last_error():

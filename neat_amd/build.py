@@ -41,14 +41,24 @@ def _run(cmd):
     return out
 
 
+def _source_id(kind):
+    sys.path.insert(0, HERE)
+    try:
+        import buildinfo
+    finally:
+        sys.path.pop(0)
+    return buildinfo.source_id(kind)
+
+
 def build_abi(jobs=8):
     os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(f for f in os.listdir(KDIR) if f.endswith(".hip"))
     objs = []
+    bid = f"-DNEAT_BUILD_ID=id{_source_id('abi')}"
 
     def compile_one(f):
         o = os.path.join(OBJ, f[:-4] + ".o")
-        _run([HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(f, []), "-c", os.path.join(KDIR, f), "-o", o])
+        _run([HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(f, []), bid, "-c", os.path.join(KDIR, f), "-o", o])
         return o
 
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -73,7 +83,8 @@ def build_host(jobs=8):
         name="_neat_host",
         sources=srcs,
         extra_include_paths=[os.path.join(ROOT, "include"), HDIR, "/opt/rocm/include"],
-        extra_cflags=["-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__=1", "-Wno-unused-variable"],
+        extra_cflags=["-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__=1", "-Wno-unused-variable",
+                      f"-DNEAT_HOST_BUILD_ID=id{_source_id('host')}"],
         extra_ldflags=[f"-L{HERE}", "-lneat_amd", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN",
                        "-L" + os.path.join(os.path.dirname(ce.__file__), "..", "..", "lib"), "-lc10_hip"],
         build_directory=build_dir,

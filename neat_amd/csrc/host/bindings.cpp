@@ -38,6 +38,14 @@ void module_methods(py::class_<M, std::shared_ptr<M>>& c)
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
 {
     m.doc() = "NeAT projector path (HyperTreeBase, IndirectGridSample3D, FCBlock, GeometryExEx) on MI355X kernels";
+// the build passes the source hash as a bare token (id + 16 hex digits), stringified here
+#ifndef NEAT_HOST_BUILD_ID
+#define NEAT_HOST_BUILD_ID idunknown
+#endif
+#define NEAT_STR2_(x) #x
+#define NEAT_STR_(x) NEAT_STR2_(x)
+    // hash of the sources this module was built from (neat_amd/buildinfo.py)
+    m.def("build_id", []() { return std::string(NEAT_STR_(NEAT_HOST_BUILD_ID) + 2); });
 
     py::class_<RayList>(m, "RayList")
         .def(py::init<>())

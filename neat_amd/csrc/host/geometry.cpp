@@ -215,6 +215,7 @@ struct TVLossFn : public torch::autograd::Function<TVLossFn>
         const int64_t st[5] = {grid.stride(0), grid.stride(1), grid.stride(2), grid.stride(3), grid.stride(4)};
         auto loss = torch::empty({1}, grid.options().dtype(torch::kFloat64));
         const int64_t A = slots.size(0), C = grid.size(1), G = grid.size(2);
+        TORCH_CHECK(A <= grid.size(0), "TV loss: ", A, " active nodes for ", grid.size(0), " grid rows");
         const size_t wsb = neat_tv_workspace_bytes(A, (int32_t)C, (int32_t)G);
         auto ws = torch::empty({(int64_t)std::max<size_t>(wsb, 16)}, grid.options().dtype(torch::kUInt8));
         NEAT_CALL(neat_tv_loss_fwd(current_stream(), dptr<float>(grid), grid.size(0), (int32_t)C, (int32_t)G, st,
@@ -327,6 +328,9 @@ torch::Tensor GeometryExEx::VolumeRegularizer(int64_t seed)
 {
     auto& tp  = params->train_params;
     auto grid = explicit_grid_generator->grid_data;
+    // the reference's index_select raises for a slot past the grid's rows; so does this
+    TORCH_CHECK(tree->NumActiveNodes() <= grid.size(0), "VolumeRegularizer: ", tree->NumActiveNodes(),
+                " active nodes for a grid of ", grid.size(0), " slots (max_active_nodes)");
     torch::Tensor loss;
     if (tp.loss_tv > 0)
     {
@@ -378,8 +382,10 @@ std::tuple<torch::Tensor, torch::Tensor, torch::Tensor> HierarchicalNeuralGeomet
         // model output times the integration weight (0 outside the active nodes)
         values.slice(1, b, e).copy_((out * all.weight.slice(0, b, e).unsqueeze(1)).t());
     }
+    // a voxel in no active node keeps the first active node as its id, as the reference's does (its mask covers only
+    // the group padding, geometry.cpp:228-233); the validity flag tells the two apart
     auto valid   = all.weight.to(torch::kLong);
-    auto node_id = all.node_id * valid;
+    auto node_id = all.node_id;
     std::vector<long> vshape = shape;
     vshape.insert(vshape.begin(), num_channels);
     return {values.reshape(vshape).cpu(), node_id.reshape(shape).cpu(), valid.reshape(shape).cpu()};

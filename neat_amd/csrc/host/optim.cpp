@@ -33,6 +33,10 @@ void NeatAdam::step()
                 st.exp_avg    = torch::zeros_like(p);
                 st.exp_avg_sq = torch::zeros_like(p);
             }
+            // moments made for another shape or layout would pair the wrong elements (torch::optim::Adam raises too)
+            TORCH_CHECK(st.exp_avg.sizes() == p.sizes() && st.exp_avg.strides() == p.strides() &&
+                            st.exp_avg_sq.strides() == p.strides(),
+                        "NeatAdam: the parameter's shape or layout changed after its moments were created");
             st.step += 1;
             NEAT_CALL(neat_adam_step(current_stream(), dptr<float>(p), dptr<float>(grad), dptr<float>(st.exp_avg),
                                      dptr<float>(st.exp_avg_sq), p.numel(), g.lr, beta1, beta2, eps, weight_decay,

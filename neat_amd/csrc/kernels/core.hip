@@ -151,6 +151,14 @@ extern "C"
 
     int neat_abi_version(void) { return 1; }
 
+// the build passes the source hash as a bare token (id + 16 hex digits), stringified here
+#ifndef NEAT_BUILD_ID
+#define NEAT_BUILD_ID idunknown
+#endif
+#define NEAT_STR2_(x) #x
+#define NEAT_STR_(x) NEAT_STR2_(x)
+    const char* neat_build_id(void) { return NEAT_STR_(NEAT_BUILD_ID) + 2; }
+
     int64_t neat_tree_num_nodes(int32_t max_depth)
     {
         if (max_depth < 0 || max_depth > 9) return -1;

@@ -340,6 +340,7 @@ struct TvDims
     uint32_t blocks;          // num_active * blocks_per_node; the grid is rounded up to a multiple of kXcds
     bool channels_last;
     bool dense;  // offset of entry e within a slot is e
+    int64_t N;   // grid slots: a slot outside [0, N) reads and writes nothing
 };
 
 // q = e / d, rem = e % d for 0 <= e < 2^24 and 1 <= d < 256 with M = ceil(2^32 / d): the round-up multiply of
@@ -390,7 +391,13 @@ __global__ void __launch_bounds__(kRedThreads) k_tv_fwd(const float* __restrict_
     }
     const int64_t a  = L / D.blocks_per_node;
     const int32_t e0 = (int32_t)(L - a * D.blocks_per_node) * (kRedThreads * kTvPerThread) + threadIdx.x;
-    const float* g0  = grid + slots[a] * st[0];
+    const int64_t sl = slots[a];
+    if (sl < 0 || sl >= D.N)
+    {
+        if (threadIdx.x == 0) partials[L] = 0.0;  // no grid row: nothing to read
+        return;
+    }
+    const float* g0  = grid + sl * st[0];
     const int32_t G  = D.G;
     // every load first (a missing neighbour reads the entry itself: difference 0), then the sums
     float v[kTvPerThread][4];
@@ -444,7 +451,13 @@ __global__ void __launch_bounds__(kRedThreads) k_tv_fwd_bwd(const float* __restr
     }
     const int64_t a  = L / D.blocks_per_node;
     const int32_t e0 = (int32_t)(L - a * D.blocks_per_node) * (kRedThreads * kTvPerThread) + threadIdx.x;
-    const int64_t b0 = slots[a] * st[0];
+    const int64_t sl = slots[a];
+    if (sl < 0 || sl >= D.N)
+    {
+        if (kLoss && threadIdx.x == 0) partials[L] = 0.0;  // no grid row: nothing to read or write
+        return;
+    }
+    const int64_t b0 = sl * st[0];
     const float* g0  = grid + b0;
     float* d0        = grad_grid + b0;
     const int32_t G  = D.G;
@@ -507,9 +520,11 @@ static Strides5 strides5(const int64_t* st) { return Strides5{{st[0], st[1], st[
 
 static int64_t xcd_grid(int64_t blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
 
-static TvDims tv_dims(int32_t C, int32_t G, const int64_t* strides, int entries_per_block, int64_t num_active)
+static TvDims tv_dims(int32_t C, int32_t G, const int64_t* strides, int entries_per_block, int64_t num_active,
+                      int64_t N)
 {
     TvDims D;
+    D.N               = N;
     D.C               = C;
     D.G               = G;
     D.MC              = C >= 2 ? (uint32_t)((((uint64_t)1 << 32) + C - 1) / C) : 0u;
@@ -661,9 +676,11 @@ extern "C"
             return NEAT_OK;
         }
         NEAT_REQUIRE(grid && slots, "neat_tv_loss_fwd: null pointer");
+        NEAT_REQUIRE(num_active <= N, "neat_tv_loss_fwd: %lld active nodes for %lld grid rows", (long long)num_active,
+                     (long long)N);
         NEAT_REQUIRE(C * (int64_t)G * G * G < (1 << 24) && C < 256 && G < 256,
                      "neat_tv_loss_fwd: C * G^3 must be below 2^24, C and G below 256");
-        const TvDims D   = tv_dims(C, G, strides, kRedThreads * kTvPerThread, num_active);
+        const TvDims D   = tv_dims(C, G, strides, kRedThreads * kTvPerThread, num_active, N);
         const int64_t nb = xcd_grid(D.blocks);
         NEAT_REQUIRE(workspace && workspace_bytes >= neat_tv_workspace_bytes(num_active, C, G),
                      "neat_tv_loss_fwd: workspace too small");
@@ -684,9 +701,11 @@ extern "C"
         NEAT_REQUIRE(strides && C > 0 && G >= 2 && num_active >= 0 && N >= 0, "neat_tv_loss_bwd: bad arguments");
         if (num_active == 0) return NEAT_OK;
         NEAT_REQUIRE(grid && slots && grad_loss && grad_grid, "neat_tv_loss_bwd: null pointer");
+        NEAT_REQUIRE(num_active <= N, "neat_tv_loss_bwd: %lld active nodes for %lld grid rows", (long long)num_active,
+                     (long long)N);
         NEAT_REQUIRE(C * (int64_t)G * G * G < (1 << 24) && C < 256 && G < 256,
                      "neat_tv_loss_bwd: C * G^3 must be below 2^24, C and G below 256");
-        const TvDims D     = tv_dims(C, G, strides, kRedThreads * kTvPerThread, num_active);
+        const TvDims D     = tv_dims(C, G, strides, kRedThreads * kTvPerThread, num_active, N);
         const double scale = weight * C / ((double)num_active * C * G * G * (G - 1));
         // the combined pass without its loss: same per-entry gradient, all of a thread's loads issued first
         if (D.dense)
@@ -713,11 +732,13 @@ extern "C"
             return NEAT_OK;
         }
         NEAT_REQUIRE(grid && slots && grad_loss && grad_grid, "neat_tv_loss_fwd_bwd: null pointer");
+        NEAT_REQUIRE(num_active <= N, "neat_tv_loss_fwd_bwd: %lld active nodes for %lld grid rows",
+                     (long long)num_active, (long long)N);
         NEAT_REQUIRE(C * (int64_t)G * G * G < (1 << 24) && C < 256 && G < 256,
                      "neat_tv_loss_fwd_bwd: C * G^3 must be below 2^24, C and G below 256");
         NEAT_REQUIRE(workspace && workspace_bytes >= neat_tv_workspace_bytes(num_active, C, G),
                      "neat_tv_loss_fwd_bwd: workspace too small");
-        const TvDims D     = tv_dims(C, G, strides, kRedThreads * kTvPerThread, num_active);
+        const TvDims D     = tv_dims(C, G, strides, kRedThreads * kTvPerThread, num_active, N);
         const int64_t nb   = xcd_grid(D.blocks);
         const double scale = weight * C / ((double)num_active * C * G * G * (G - 1));
         double* part       = static_cast<double*>(workspace);

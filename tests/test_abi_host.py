@@ -107,3 +107,17 @@ def test_workspace_queries_scale():
     assert L.neat_decoder_workspace_bytes(1 << 20, 8, 64, 1, 1) >= 4 * 4801
     assert L.neat_decoder_num_params(8, 64, 1, 1) == 4801
     assert L.neat_decoder_num_params(8, 32, 1, 1) == 1377
+
+
+def test_tv_loss_refuses_more_active_nodes_than_grid_rows():
+    """The total variation reads and writes the grid rows its slots name: more active nodes than rows is an argument
+    error, returned before any launch (the reference's index_select raises there too)."""
+    L = _abi.lib()
+    st = np.array([8 * 27, 1, 8 * 9, 8 * 3, 8], np.int64)
+    dummy = ctypes.c_void_p(8)
+    ws = ctypes.c_void_p(256)
+    assert L.neat_tv_loss_fwd(None, dummy, 4, 8, 3, st.ctypes.data, dummy, 5, 1.0, dummy, ws, 1 << 20) == NEAT_E_ARG
+    assert "5 active nodes for 4 grid rows" in _abi.last_error()
+    assert L.neat_tv_loss_bwd(None, dummy, 4, 8, 3, st.ctypes.data, dummy, 5, 1.0, dummy, dummy) == NEAT_E_ARG
+    assert L.neat_tv_loss_fwd_bwd(None, dummy, 4, 8, 3, st.ctypes.data, dummy, 5, 1.0, dummy, dummy, dummy, 1, ws,
+                                  1 << 20) == NEAT_E_ARG

@@ -60,7 +60,8 @@ def test_uniform_sampled_volume_vs_oracle(H, oracle):
     node = R.active_node_at(sc["tree"], sc["active"], coord)
     ok = node >= 0
     assert np.array_equal(valid.numpy().reshape(-1), ok.astype(np.int64))
-    assert np.array_equal(vnode.numpy().reshape(-1), np.where(ok, node, 0))
+    # outside every active node: the first active node, as the reference writes it (geometry.cpp:228-233)
+    assert np.array_equal(vnode.numpy().reshape(-1), np.where(ok, node, sc["active_ids"][0]))
     uv = oracle.local_samples(sc["tree"], coord[ok], node[ok])
     f = oracle.grid_sample_fwd(sc["grid"].astype(np.float64), sc["prefix"][node[ok]], uv.astype(np.float64))
     y = oracle.decoder_fwd(f, sc["params"], cfg.channels, cfg.hidden, cfg.layers, 1, act=cfg.act,
