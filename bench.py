@@ -56,6 +56,10 @@ def parse():
                     help="add the reference training step's volume regulariser (total variation of the active grids "
                          "at loss_tv 1e-4 and the edge loss along neighbouring leaves' faces at loss_edge 1e-3, with "
                          "its grid gradient) to every step, timed as its own stage")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the in-process comparison forms (FP32 on the matrix cores, the first FP32 versions) and "
+                         "the timed training step")
+    ap.add_argument("--compare-steps", type=int, default=3, help="timed steps of each comparison form")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference kernel for scaling HBM "
                          "byte counters) and exit")
@@ -74,6 +78,132 @@ def copy_calibration(mib):
     return 0
 
 
+class Runner:
+    """One projector form over the resident scene: (path, precision, proposal sampler or None). step(i, timed) runs
+    one fwd+bwd pass over batch i % batches with per-stage HIP events on the launch stream when timed; with
+    `train`, an Adam step on the grid (lr 1e-2) and the decoder (lr 1e-3) and the BF16 grid refresh follow
+    (geometry_ex_ex.cpp:40-49, geometry.h:66-88)."""
+
+    def __init__(self, a, cfg, ds, batches, path, precision, prop=None, grid_bf16=None, train=False,
+                 active_ids=None):
+        import torch
+        self.a, self.cfg, self.ds, self.batches = a, cfg, ds, batches
+        self.path, self.precision, self.prop, self.grid_bf16, self.train = path, precision, prop, grid_bf16, train
+        self.active_ids = active_ids
+        names = (["sample", "plan", "proj_fwd", "loss", "bwd_decoder", "bwd_params", "bwd_grid"]
+                 if path == "fused" else ["sample", "gs_fwd", "dec_fwd", "integrate", "dec_bwd", "gs_bwd"])
+        if prop is not None:
+            names = ["sample", "proposal", "proposal_train"] + names[1:]
+        if a.regularizer and path == "fused":
+            names = names + ["regularizer"]
+        if train:
+            names = names + ["optimizer"]
+        self.stage_names = names
+        self.events = {k: [] for k in names}
+        self.gg_buf = torch.empty_like(ds.grid)
+        self.gp_buf = torch.empty_like(ds.params)
+        if train:
+            self.m_grid, self.v_grid = torch.zeros_like(ds.grid), torch.zeros_like(ds.grid)
+            self.m_par, self.v_par = torch.zeros_like(ds.params), torch.zeros_like(ds.params)
+            self.opt_steps = 0
+
+    def step(self, i, timed):
+        import torch
+        from neat_amd import ops, pipeline
+        a, ds = self.a, self.ds
+        o, d, tgt = self.batches[i % len(self.batches)]
+        ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for k in self.stage_names}
+        rec = (lambda k, j: ev[k][j].record()) if timed else (lambda k, j: None)
+        if self.path == "fused":
+            rec("sample", 0)
+            segs = pipeline.sample(ds, o, d, jitter=True, seed=i, emit=False)
+            rec("sample", 1)
+            if self.prop is not None:
+                rec("proposal", 0)
+                segs = self.prop.apply(o, d, segs, ds.prefix)
+                rec("proposal", 1)
+                rec("proposal_train", 0)
+                if i % a.proposal_train_every == 0:
+                    so, sd = o[::32].contiguous(), d[::32].contiguous()
+                    self.prop.train_step(ds, so, sd, pipeline.sample(ds, so, sd, jitter=False, emit=False),
+                                         precision="bf16", grid_bf16=self.grid_bf16)
+                rec("proposal_train", 1)
+            rec("plan", 0)
+            pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, o, d, segs, ds.grid, ds.params, ds.mlp, True, i,
+                               precision=self.precision, grid_bf16=self.grid_bf16)
+            rec("plan", 1)
+            rec("proj_fwd", 0)
+            proj = pj.forward()
+            rec("proj_fwd", 1)
+            rec("loss", 0)
+            loss, gproj = ops.mse_fp64(proj, tgt)
+            rec("loss", 1)
+            # the backward's three stages (decoder, parameter gradient, grid gradient), timed one by one
+            for st, name in ((1, "bwd_decoder"), (2, "bwd_params"), (3, "bwd_grid")):
+                rec(name, 0)
+                pj.backward_stage(st, gproj, self.gg_buf, self.gp_buf)
+                rec(name, 1)
+            if a.regularizer:
+                rec("regularizer", 0)
+                ops.volume_regularizer(ds.pos_min, ds.pos_max, ds.flags, ds.prefix, self.active_ids, ds.grid, seed=i)
+                rec("regularizer", 1)
+            nsamp = pj.S
+        else:
+            R = o.shape[0]
+            rec("sample", 0)
+            smp = pipeline.sample(ds, o, d, jitter=True, seed=i)
+            rec("sample", 1)
+            uv = ops.local_samples(smp["coord"], smp["node_id"], ds.pos_min, ds.pos_max)
+            slot = ops.node_to_slot(smp["node_id"], ds.prefix)
+            rec("gs_fwd", 0)
+            feat = ops.gs3d_fwd(ds.grid, slot, uv)
+            rec("gs_fwd", 1)
+            rec("dec_fwd", 0)
+            y = ops.decoder_fwd(feat, ds.params, *pipeline._mlp_args(ds))
+            rec("dec_fwd", 1)
+            rec("integrate", 0)
+            proj = ops.integrate_fwd(y, smp["weight"], smp["ray_index"], R, smp["ray_sample_offset"])
+            loss, gproj = ops.mse_fp64(proj.reshape(-1), tgt)
+            gy = ops.integrate_bwd(gproj.reshape(1, R), smp["weight"], smp["ray_index"])
+            rec("integrate", 1)
+            rec("dec_bwd", 0)
+            gparams, gfeat = ops.decoder_bwd(feat, ds.params, gy, *pipeline._mlp_args(ds))
+            rec("dec_bwd", 1)
+            rec("gs_bwd", 0)
+            ggrid, _ = ops.gs3d_bwd(ds.grid, slot, uv, gfeat, want_uv=False)
+            rec("gs_bwd", 1)
+            self.gg_buf, self.gp_buf = ggrid, gparams
+            nsamp = int(smp["ray_sample_offset"][R].item())
+        if self.train:
+            rec("optimizer", 0)
+            self.opt_steps += 1
+            ops.adam_step(ds.grid, self.gg_buf, self.m_grid, self.v_grid, 1e-2, self.opt_steps)
+            ops.adam_step(ds.params, self.gp_buf, self.m_par, self.v_par, 1e-3, self.opt_steps)
+            if self.grid_bf16 is not None:
+                ops.grid_to_bf16(ds.grid, out=self.grid_bf16)
+            rec("optimizer", 1)
+        if timed:
+            for k in self.stage_names:
+                self.events[k].append(ev[k])
+        return nsamp
+
+    def run(self, steps, warmup):
+        """warmup untimed steps, then `steps` timed between two device synchronisations: (ms per step, mean samples
+        per batch, mean ms per stage)."""
+        import torch
+        for i in range(warmup):
+            self.step(i, False)
+        torch.cuda.synchronize()
+        samples = []
+        t0 = time.perf_counter()
+        for i in range(steps):
+            samples.append(self.step(warmup + i, True))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        stage_ms = {k: float(np.mean([s.elapsed_time(e) for s, e in self.events[k]])) for k in self.stage_names}
+        return (t1 - t0) * 1e3 / steps, float(np.mean(samples)), stage_ms
+
+
 def main():
     a = parse()
     if a.sampling == "auto":
@@ -86,7 +216,7 @@ def main():
         return 2
     import torch
 
-    from neat_amd import ops, pipeline, workloads as W
+    from neat_amd import _abi, host, ops, pipeline, workloads as W
 
     cfg = W.C2
     dev = "cuda"
@@ -105,12 +235,9 @@ def main():
 
     # the BF16 grid copy is refreshed once per optimizer step in training; here the grid is fixed, so once
     grid_bf16 = ops.grid_to_bf16(ds.grid) if (a.path == "fused" and a.precision == "bf16") else None
-    stage_names = (["sample", "plan", "proj_fwd", "loss", "bwd_decoder", "bwd_params", "bwd_grid"]
-                   if a.path == "fused" else ["sample", "gs_fwd", "dec_fwd", "integrate", "dec_bwd", "gs_bwd"])
     prop = None
     if a.proposal:
         from neat_amd import proposal as PP
-        stage_names = ["sample", "proposal", "proposal_train"] + stage_names[1:]
         prop = PP.ProposalSampler(dev, num_samples=8, lr=3e-3, seed=a.seed + 7, num_slots=ds.grid.shape[0],
                                   tau=a.proposal_tau)
         for k in range(a.proposal_pretrain):  # untimed set-up: the network starts trained
@@ -119,102 +246,11 @@ def main():
             prop.train_step(ds, to, td, pipeline.sample(ds, to, td, jitter=False, emit=False), precision="bf16",
                             grid_bf16=grid_bf16)
         torch.cuda.synchronize()
-    if a.regularizer:
-        stage_names = stage_names + ["regularizer"]
-        active_ids = torch.from_numpy(scene["active_ids"]).to(dev)
-    events = {k: [] for k in stage_names}
-    gg_buf = torch.empty_like(ds.grid)
-    gp_buf = torch.empty_like(ds.params)
-
-    def fused_step(i, timed):
-        o, d, tgt = batches[i % len(batches)]
-        ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for k in stage_names}
-        rec = (lambda k, j: ev[k][j].record()) if timed else (lambda k, j: None)
-        rec("sample", 0)
-        segs = pipeline.sample(ds, o, d, jitter=True, seed=i, emit=False)
-        rec("sample", 1)
-        if prop is not None:
-            rec("proposal", 0)
-            segs = prop.apply(o, d, segs, ds.prefix)
-            rec("proposal", 1)
-            rec("proposal_train", 0)
-            if i % a.proposal_train_every == 0:
-                so, sd = o[::32].contiguous(), d[::32].contiguous()
-                prop.train_step(ds, so, sd, pipeline.sample(ds, so, sd, jitter=False, emit=False), precision="bf16",
-                                grid_bf16=grid_bf16)
-            rec("proposal_train", 1)
-        rec("plan", 0)
-        pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, o, d, segs, ds.grid, ds.params, ds.mlp, True, i,
-                           precision=a.precision, grid_bf16=grid_bf16)
-        rec("plan", 1)
-        rec("proj_fwd", 0)
-        proj = pj.forward()
-        rec("proj_fwd", 1)
-        rec("loss", 0)
-        loss, gproj = ops.mse_fp64(proj, tgt)
-        rec("loss", 1)
-        # the backward's three stages (decoder, parameter gradient, grid gradient), timed one by one
-        for st, name in ((1, "bwd_decoder"), (2, "bwd_params"), (3, "bwd_grid")):
-            rec(name, 0)
-            pj.backward_stage(st, gproj, gg_buf, gp_buf)
-            rec(name, 1)
-        if a.regularizer:
-            rec("regularizer", 0)
-            ops.volume_regularizer(ds.pos_min, ds.pos_max, ds.flags, ds.prefix, active_ids, ds.grid, seed=i)
-            rec("regularizer", 1)
-        if timed:
-            for k in stage_names:
-                events[k].append(ev[k])
-        return pj.S
-
-    def step(i, timed):
-        if a.path == "fused":
-            return fused_step(i, timed)
-        o, d, tgt = batches[i % len(batches)]
-        R = o.shape[0]
-        ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for k in stage_names}
-        rec = (lambda k, j: ev[k][j].record()) if timed else (lambda k, j: None)
-        rec("sample", 0)
-        smp = pipeline.sample(ds, o, d, jitter=True, seed=i)
-        rec("sample", 1)
-        uv = ops.local_samples(smp["coord"], smp["node_id"], ds.pos_min, ds.pos_max)
-        slot = ops.node_to_slot(smp["node_id"], ds.prefix)
-        rec("gs_fwd", 0)
-        feat = ops.gs3d_fwd(ds.grid, slot, uv)
-        rec("gs_fwd", 1)
-        rec("dec_fwd", 0)
-        y = ops.decoder_fwd(feat, ds.params, *pipeline._mlp_args(ds))
-        rec("dec_fwd", 1)
-        rec("integrate", 0)
-        proj = ops.integrate_fwd(y, smp["weight"], smp["ray_index"], R, smp["ray_sample_offset"])
-        loss, gproj = ops.mse_fp64(proj.reshape(-1), tgt)
-        gy = ops.integrate_bwd(gproj.reshape(1, R), smp["weight"], smp["ray_index"])
-        rec("integrate", 1)
-        rec("dec_bwd", 0)
-        gparams, gfeat = ops.decoder_bwd(feat, ds.params, gy, *pipeline._mlp_args(ds))
-        rec("dec_bwd", 1)
-        rec("gs_bwd", 0)
-        ggrid, _ = ops.gs3d_bwd(ds.grid, slot, uv, gfeat, want_uv=False)
-        rec("gs_bwd", 1)
-        if timed:
-            for k in stage_names:
-                events[k].append(ev[k])
-        return int(smp["ray_sample_offset"][R].item())
-
-    for i in range(a.warmup):
-        step(i, False)
-    torch.cuda.synchronize()
-    samples = []
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        samples.append(step(a.warmup + i, True))
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / a.steps
+    active_ids = torch.from_numpy(scene["active_ids"]).to(dev) if a.regularizer else None
+    head = Runner(a, cfg, ds, batches, a.path, a.precision, prop, grid_bf16, active_ids=active_ids)
+    ms_per_step, mean_samples, stage_ms = head.run(a.steps, a.warmup)
     rays_per_s = cfg.batch / (ms_per_step / 1e3)
-    mean_samples = float(np.mean(samples))
 
-    stage_ms = {k: float(np.mean([s.elapsed_time(e) for s, e in events[k]])) for k in stage_names}
     dominant = max(stage_ms, key=stage_ms.get)
     C, H = cfg.channels, cfg.hidden
     fwd_flop = 2 * (C * H + H * H + H * 1)  # decoder forward per sample, 2 FLOP per MAC
@@ -229,12 +265,13 @@ def main():
         bwd_kernel = f"k_mfma_bwd_relu<{H}>"
     else:
         bwd_kernel = f"k_mfma_bwd<{prec_tag},{H},{act}>"
+    fwd_kernel = (f"k_mfma_fwd_relu<{H}>" if prec_tag == "PrecBF16" and cfg.act == "relu" else
+                  f"k_mfma_fwd<{prec_tag},{H},{act}>" if mm and prec_tag else f"k_proj_fwd<{C},{H},{act}>")
     # algorithmic work per launch of each stage's dominant kernel (per-unit figure x the units of one launch):
-    # decoder FLOPs per sample (backward: 2x the forward, no recomputation counted); the scatter moves 48 B per
-    # sample (grid coordinate 16 B + dL/dfeature 32 B) and writes the gradient grid once
+    # decoder FLOPs per sample (backward: 2x the forward, no recomputation counted); the scatter reads 16 B of
+    # dL/dfeature per sample and writes the gradient grid once
     work = {
-        "proj_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", mm_peak,
-                     f"k_mfma_fwd<{prec_tag},{H},{act}>" if mm and prec_tag else f"k_proj_fwd<{C},{H},{act}>"),
+        "proj_fwd": ("mfma", fwd_flop * mean_samples, "TFLOP/s", mm_peak, fwd_kernel),
         "bwd_decoder": ("mfma", 2 * fwd_flop * mean_samples, "TFLOP/s", mm_peak, bwd_kernel),
         "bwd_grid": ("hbm", 48 * mean_samples + grid_bytes, "GB/s", HBM_PEAK_GBS, f"k_proj_bw2<{C}>"),
         "bwd_params": ("hbm", 4.0 * 1024 * ds.params.numel(), "GB/s", HBM_PEAK_GBS, "k_dec_reduce_partials"),
@@ -260,12 +297,54 @@ def main():
                 "traffic": measured_traffic(workload_name(a), kname), "kernel": kname,
                 "stage": dominant, "kernel_ms": round(stage_ms[dominant], 4)}
 
+    err = projection_error(scene, cfg, ds, prop, a, grid_bf16)
+
+    # comparison forms on the same field and rays, after the timed region: the FP32 forms sample uniformly (the
+    # proposal is part of the BF16 form); the first plain-FP32 GPU version (the unfused kernel chain) is the
+    # denominator the north-star target is stated against
+    baselines, vs_baseline, train = None, None, None
+    if not a.no_compare and a.path == "fused" and a.precision == "bf16" and a.proposal and not a.regularizer:
+        baselines = {}
+        for name, path, prec in (("fp32_mfma", "fused", "fp32"), ("fp32_v1_fused_valu", "fused", "fp32_valu"),
+                                 ("fp32_v1_unfused", "unfused", "fp32")):
+            r = Runner(a, cfg, ds, batches, path, prec)
+            ms, ns, _ = r.run(a.compare_steps, 1)
+            baselines[name] = {"rays_per_s": round(cfg.batch / (ms / 1e3), 1), "ms_per_step": round(ms, 3),
+                               "samples_per_batch": round(ns), "ratio": round(rays_per_s / (cfg.batch / (ms / 1e3)), 2)}
+            del r
+            torch.cuda.empty_cache()
+        vs_baseline = baselines["fp32_v1_unfused"]["ratio"]
+        baselines["denominator"] = "fp32_v1_unfused (the first plain-FP32 GPU version: the unfused kernel chain)"
+        # the training step: the headline form followed by Adam on the grid and the decoder and the BF16 refresh,
+        # on copies of the field (the headline's tensors stay as they were)
+        tds = pipeline.DeviceScene(ds.max_depth, ds.pos_min, ds.pos_max, ds.flags, ds.prefix, ds.grid.clone(),
+                                   ds.params.clone(), ds.mlp, ds.max_spn)
+        tr = Runner(a, cfg, tds, batches, "fused", "bf16", prop, ops.grid_to_bf16(tds.grid), train=True)
+        ms, ns, sm = tr.run(a.steps, a.warmup)
+        train = {"rays_per_s": round(cfg.batch / (ms / 1e3), 1), "ms_per_step": round(ms, 3),
+                 "optimizer_ms": round(sm["optimizer"], 3),
+                 "what": "headline fwd+bwd, then Adam on the grid (lr 1e-2) and decoder (lr 1e-3) and the BF16 grid "
+                         "refresh"}
+        del tr, tds
+        torch.cuda.empty_cache()
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(scene, cfg, a.cpu_rays, a.seed)
-    err = projection_error(scene, cfg, ds, prop, a, grid_bf16)
 
     prec_name = workload_name(a)
+    config = {"workload": f"projector fwd+bwd ({prec_name}): depth-5 octree, 4096 adaptive leaves, 17^3x8 grids, "
+                          "decoder 8->64->64->1, 256x512^2 cone beam, batch 2^18 rays, max 64 samples/node",
+              "path": a.path, "precision": prec_name, "batch_rays": cfg.batch,
+              "samples_per_batch": round(mean_samples),
+              "samples_per_s": round(mean_samples / (ms_per_step / 1e3), 1),
+              "field": "phantom fitted on the leaf grids" if prop is not None else "random grid N(0, 0.1)",
+              "projection_error_vs_fp64": err,
+              "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
+              "build_id": {"libneat_amd": _abi.build_id(), "neat_host": host.load().build_id()}}
+    if baselines is not None:
+        config["baselines"] = baselines
+        config["train_step"] = train
     out = {
         "metric": "fwd+bwd rays/s at 1 GPU, BF16+proposal vs plain FP32, at <=1e-3 rel err vs FP64",
         "value": round(rays_per_s, 1),
@@ -276,16 +355,10 @@ def main():
         "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": vs_baseline,
         "dtype": prec_name.split("_")[0],
         "data": "synthetic",
-        "config": {"workload": f"projector fwd+bwd ({prec_name}): depth-5 octree, 4096 adaptive leaves, 17^3x8 grids, "
-                               "decoder 8->64->64->1, 256x512^2 cone beam, batch 2^18 rays, max 64 samples/node",
-                   "path": a.path, "precision": prec_name, "batch_rays": cfg.batch, "samples_per_batch": round(mean_samples),
-                   "samples_per_s": round(mean_samples / (ms_per_step / 1e3), 1),
-                   "field": "phantom fitted on the leaf grids" if prop is not None else "random grid N(0, 0.1)",
-                   "projection_error_vs_fp64": err,
-                   "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}},
+        "config": config,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -341,25 +414,57 @@ def measured_traffic(workload, kernel):
     return round(1024 * (k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"]))
 
 
-def cpu_baseline(scene, cfg, num_rays, seed):
-    """The oracle's FP32 restatement of the reference path (sampling by brute force over the active leaves, then
-    the composed forward and backward), single-threaded like the reference's CPU loops, on a bounded ray sample."""
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from neat_amd import workloads as W
+def host_cores():
+    """Host threads this process may use: its CPU affinity, capped by OMP_NUM_THREADS when that is set (the GPU box
+    gives each GPU a share of the host)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def _cpu_pass(scene, cfg, o, d, seed, nthreads):
+    """The oracle's FP32 restatement of the reference path on rays (o, d): sampling by brute force over the active
+    leaves, then the composed forward and backward; returns the elapsed seconds."""
     from oracle import oracle as O
-    o, d = W.batch_rays(cfg, num_rays, seed=seed + 999)
     tgt = scene["phantom"].line_integrals(o, d)
     t0 = time.perf_counter()
-    segs = O.raysample(scene["tree"], scene["active_ids"], o, d, cfg.max_spn, emit=False, nthreads=1)
+    segs = O.raysample(scene["tree"], scene["active_ids"], o, d, cfg.max_spn, emit=False, nthreads=nthreads)
     proj = O.projector_fwd(scene["tree"], scene["prefix"], o, d, segs, scene["grid"], dict(scene["mlp"],
-                           params=scene["params"]), jitter=True, seed=seed, prec=0, nthreads=1)
-    gproj = 2 * (proj - tgt) / num_rays
+                           params=scene["params"]), jitter=True, seed=seed, prec=0, nthreads=nthreads)
+    gproj = 2 * (proj - tgt) / o.shape[0]
     O.projector_bwd(scene["tree"], scene["prefix"], o, d, segs, scene["grid"], dict(scene["mlp"],
-                    params=scene["params"]), gproj, jitter=True, seed=seed, prec=0, nthreads=1)
-    dt = time.perf_counter() - t0
-    return {"value": round(num_rays / dt, 2), "unit": "rays/s", "cores": 1, "kind": "port",
-            "sample": f"{num_rays} rays of the same workload (sampling + fwd + bwd), single-threaded oracle, "
-                      f"{dt:.1f} s"}
+                    params=scene["params"]), gproj, jitter=True, seed=seed, prec=0, nthreads=nthreads)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(scene, cfg, num_rays, seed):
+    """The reference's CPU path, restated (the oracle), in the two variants the baseline plan names: (i) one thread,
+    like the reference's CPU loops, on a bounded sample of the benchmarked configuration; (ii) ray-parallel forward
+    and thread-privatised backward over every host core this process may use, on a larger sample of the same
+    configuration and on one full 2^18-ray batch of the small configuration (depth 2, 64 leaves)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from neat_amd import workloads as W
+    cores = host_cores()
+    o, d = W.batch_rays(cfg, num_rays, seed=seed + 999)
+    dt1 = _cpu_pass(scene, cfg, o, d, seed, 1)
+    n_all = num_rays * max(1, min(cores, 16)) // 2
+    o, d = W.batch_rays(cfg, n_all, seed=seed + 998)
+    dtn = _cpu_pass(scene, cfg, o, d, seed, cores)
+    c1 = W.C1
+    sc1 = W.make_scene(c1, seed=seed)
+    o, d = W.batch_rays(c1, c1.batch, seed=seed + 997)
+    dtc1 = _cpu_pass(sc1, c1, o, d, seed, cores)
+    return {"value": round(n_all / dtn, 2), "unit": "rays/s", "cores": cores, "kind": "port",
+            "sample": f"{n_all} rays of the same workload (sampling + fwd + bwd), oracle over {cores} threads "
+                      f"(ray-parallel forward, thread-privatised backward), {dtn:.1f} s",
+            "single_thread": {"value": round(num_rays / dt1, 2), "unit": "rays/s", "cores": 1,
+                              "sample": f"{num_rays} rays of the same workload, one thread, {dt1:.1f} s"},
+            "c1_full_batch": {"value": round(c1.batch / dtc1, 2), "unit": "rays/s", "cores": cores,
+                              "seconds": round(dtc1, 2),
+                              "sample": f"one full batch of {c1.batch} rays of the small configuration (depth-2 "
+                                        f"octree, 64 leaves, 11^3x8 grids, decoder 8->32->32->1), {cores} threads"}}
 
 
 if __name__ == "__main__":

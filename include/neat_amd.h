@@ -107,6 +107,24 @@ int neat_raysample_segments(neat_stream_t stream, const float* node_pos_min, con
                             int32_t* seg_node, int32_t* seg_ray, int32_t* seg_count, float* seg_tmin, float* seg_tmax,
                             int64_t* seg_sample_offset);
 
+/* Single-walk form of passes 1 and 2 (the path the module layer takes): neat_raysample_walk walks every ray once,
+ * writes the counts and offsets as neat_raysample_count does, and keeps each ray's canonical segments in a caller
+ * scratch of neat_raysample_scratch_bytes(num_rays, max_depth) bytes (16-byte aligned); neat_raysample_compact then
+ * copies them into the canonical segment list as neat_raysample_segments writes it (a ray with more segments than
+ * the scratch holds per ray is walked again there). Same replacement as above (HyperTree.cu:206-438). */
+size_t neat_raysample_scratch_bytes(int64_t num_rays, int32_t max_depth);
+int neat_raysample_walk(neat_stream_t stream, const float* node_pos_min, const float* node_pos_max,
+                        const uint8_t* node_flags, int32_t num_nodes, int32_t max_depth, const float* ray_origin,
+                        const float* ray_direction, int64_t num_rays, int32_t max_samples_per_node,
+                        int64_t* ray_seg_offset, int64_t* ray_sample_offset, int32_t* max_samples_per_ray_out,
+                        void* scratch, size_t scratch_bytes, void* workspace, size_t workspace_bytes);
+int neat_raysample_compact(neat_stream_t stream, const float* node_pos_min, const float* node_pos_max,
+                           const uint8_t* node_flags, int32_t num_nodes, int32_t max_depth, const float* ray_origin,
+                           const float* ray_direction, int64_t num_rays, int32_t max_samples_per_node,
+                           const int64_t* ray_seg_offset, const int64_t* ray_sample_offset, const void* scratch,
+                           size_t scratch_bytes, int64_t num_segments, int32_t* seg_node, int32_t* seg_ray,
+                           int32_t* seg_count, float* seg_tmin, float* seg_tmax, int64_t* seg_sample_offset);
+
 /* Pass 3: SampleList columns (HelperStructs.h:168-189) for every sample: t, weight (midpoint cell width),
  * global coordinate o + d*t [S,3] (may be NULL), ray index, index within the ray, node id. jitter != 0 draws the
  * in-cell position from the counter hash of (seed, ray, segment rank, j); jitter == 0 uses the cell midpoint. */

@@ -64,7 +64,8 @@ def build_abi(jobs=8):
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(compile_one, srcs))
     tmp = ABI_LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    # the soname lets a module that needs libneat_amd.so reuse the copy the Python loader already mapped
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-soname,libneat_amd.so", *objs, "-o", tmp])
     os.replace(tmp, ABI_LIB)
     return ABI_LIB
 
@@ -85,7 +86,7 @@ def build_host(jobs=8):
         extra_include_paths=[os.path.join(ROOT, "include"), HDIR, "/opt/rocm/include"],
         extra_cflags=["-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__=1", "-Wno-unused-variable",
                       f"-DNEAT_HOST_BUILD_ID=id{_source_id('host')}"],
-        extra_ldflags=[f"-L{HERE}", "-lneat_amd", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN",
+        extra_ldflags=[f"-L{HERE}", "-lneat_amd", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{HERE}",
                        "-L" + os.path.join(os.path.dirname(ce.__file__), "..", "..", "lib"), "-lc10_hip"],
         build_directory=build_dir,
         verbose=False,

@@ -174,11 +174,16 @@ SampleList HyperTreeBaseImpl::create_samples(const RayList& rays, int max_spn, b
     seg.ray_sample_offset = torch::empty({R + 1}, L);
     auto maxr             = torch::empty({1}, I);
     const size_t wsb      = neat_raysample_workspace_bytes(R);
-    auto ws               = torch::empty({(int64_t)std::max<size_t>(wsb, 8)}, torch::TensorOptions().dtype(torch::kUInt8).device(dev));
-    NEAT_CALL(neat_raysample_count(s, dptr<float>(node_position_min), dptr<float>(node_position_max),
-                                   node_flags.data_ptr<uint8_t>(), NumNodes(), max_depth_, dptr<float>(o),
-                                   dptr<float>(d), R, max_spn, dptr<int64_t>(seg.ray_seg_offset),
-                                   dptr<int64_t>(seg.ray_sample_offset), dptr<int32_t>(maxr), ws.data_ptr(), wsb));
+    const auto U8         = torch::TensorOptions().dtype(torch::kUInt8).device(dev);
+    auto ws               = torch::empty({(int64_t)std::max<size_t>(wsb, 8)}, U8);
+    // one walk per ray: the segments wait in a per-ray scratch until the totals size the list
+    const size_t scb      = neat_raysample_scratch_bytes(R, max_depth_);
+    auto scratch          = torch::empty({(int64_t)std::max<size_t>(scb, 16)}, U8);
+    NEAT_CALL(neat_raysample_walk(s, dptr<float>(node_position_min), dptr<float>(node_position_max),
+                                  node_flags.data_ptr<uint8_t>(), NumNodes(), max_depth_, dptr<float>(o),
+                                  dptr<float>(d), R, max_spn, dptr<int64_t>(seg.ray_seg_offset),
+                                  dptr<int64_t>(seg.ray_sample_offset), dptr<int32_t>(maxr), scratch.data_ptr(), scb,
+                                  ws.data_ptr(), wsb));
     // one device -> host read of the totals per batch (the reference syncs on its counters too, HyperTree.cu:420)
     auto totals = torch::stack({seg.ray_seg_offset[R], seg.ray_sample_offset[R], maxr[0].to(torch::kLong)}).cpu();
     const int64_t K = totals[0].item<int64_t>();
@@ -189,12 +194,12 @@ SampleList HyperTreeBaseImpl::create_samples(const RayList& rays, int max_spn, b
     seg.tmin          = torch::empty({K}, F);
     seg.tmax          = torch::empty({K}, F);
     seg.sample_offset = torch::empty({K}, L);
-    NEAT_CALL(neat_raysample_segments(s, dptr<float>(node_position_min), dptr<float>(node_position_max),
-                                      node_flags.data_ptr<uint8_t>(), NumNodes(), max_depth_, dptr<float>(o),
-                                      dptr<float>(d), R, max_spn, dptr<int64_t>(seg.ray_seg_offset),
-                                      dptr<int64_t>(seg.ray_sample_offset), K, dptr<int32_t>(seg.node_id),
-                                      dptr<int32_t>(seg.ray), dptr<int32_t>(seg.count), dptr<float>(seg.tmin),
-                                      dptr<float>(seg.tmax), dptr<int64_t>(seg.sample_offset)));
+    NEAT_CALL(neat_raysample_compact(s, dptr<float>(node_position_min), dptr<float>(node_position_max),
+                                     node_flags.data_ptr<uint8_t>(), NumNodes(), max_depth_, dptr<float>(o),
+                                     dptr<float>(d), R, max_spn, dptr<int64_t>(seg.ray_seg_offset),
+                                     dptr<int64_t>(seg.ray_sample_offset), scratch.data_ptr(), scb, K,
+                                     dptr<int32_t>(seg.node_id), dptr<int32_t>(seg.ray), dptr<int32_t>(seg.count),
+                                     dptr<float>(seg.tmin), dptr<float>(seg.tmax), dptr<int64_t>(seg.sample_offset)));
     seg.num_samples   = S;
     seg.jitter        = jitter;
     seg.seed          = jitter ? (uint64_t)torch::randint(0, (int64_t)1 << 62, {1}, torch::kLong).item<int64_t>() : 0;

@@ -141,6 +141,37 @@ __global__ void __launch_bounds__(256) k_plan_chunks(ProjGeom g, Plan p)
     (pass1 ? p.chunk1_q1 : p.chunk2_q1)[i]     = q1;
 }
 
+// Per-segment records in node-major order (make_record), after the sort.
+__global__ void __launch_bounds__(256) k_plan_records(ProjGeom g, Plan p)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < g.K) make_record(g, p, q);
+}
+
+// Node-resident workgroup ranges: workgroup b takes the forward chunks whose first segment starts at a cost
+// (samples + kSegCost x segments, over the node-major list) in [b T / nwg, (b + 1) T / nwg), T the total cost;
+// thread b writes the first chunk of workgroup b (b = nwg: the chunk count).
+__global__ void __launch_bounds__(256) k_plan_wg(ProjGeom g, Plan p)
+{
+    const int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > p.nwg) return;
+    const int64_t nch    = p.chunk_off1[g.N];
+    const int32_t kv     = p.slot_begin[g.N];
+    const int64_t total  = p.nm_soff[kv] + (int64_t)kSegCost * kv;
+    const int64_t target = total * b / p.nwg;
+    int64_t lo = 0, hi = nch;
+    while (lo < hi)
+    {
+        const int64_t mid = (lo + hi) >> 1;
+        const int32_t q   = p.chunk1_q0[mid];
+        if (p.nm_soff[q] + (int64_t)kSegCost * q < target)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    p.wg_first[b] = (int32_t)(b == p.nwg ? nch : lo);
+}
+
 // ---- forward --------------------------------------------------------------------------------------------------------
 template <int C, int H, int ACT>
 __global__ void __launch_bounds__(kFwdThreads) k_proj_fwd(ProjGeom g, Plan p, const float* __restrict__ params,
@@ -381,81 +412,100 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
         // every lane of an instruction on the same eighth of the banks).
         unsigned long long* sq = reinterpret_cast<unsigned long long*>(sg);
         const int32_t nv       = g.G * g.G * g.G;
-        const int sub = lane >> 3, kk = lane & 7;
-        const int64_t U2 = (int64_t)(blockDim.x >> 6) * 8;
+        const int32_t G        = g.G;
+        // Lane = sample stream: the chunk's samples are cut into blockDim.x contiguous streams, and a lane walks its
+        // stream in order, with all 8 corners x 4 channels of the current trilinear cell summed in FP32 registers;
+        // the cell's sums go to the fixed-point LDS grid (32 integer atomics) only when the stream leaves the cell
+        // (about every 4 samples along a ray). The cells, their runs and the order inside a run are fixed by the
+        // plan, so the grid gradient stays bit-reproducible.
+        const int64_t U2 = blockDim.x;
         const int64_t L2 = (ns + U2 - 1) / U2;
-        int64_t s2       = ((int64_t)sub * (blockDim.x >> 6) + wave) * L2;  // far-apart streams share a wave
+        int64_t s2       = (int64_t)threadIdx.x * L2;
         const int64_t e2 = min(ns, s2 + L2);
         const double scale = fixed_scale(*dmax);
         const float4* dh   = reinterpret_cast<const float4*>(dfeat) + (int64_t)blockIdx.y * g.S + base;
         if (s2 < e2)
         {
-            // Consecutive samples of a stream usually fall in the same cell (a run of about 4 samples per corner):
-            // a lane sums its corner's 4 channels over the run in FP32 registers, in stream order, and converts the
-            // run's sums to fixed point once, when the vertex changes. The runs and their order are fixed by the
-            // plan, so the grid gradient stays bit-reproducible. Loads go in blocks of kSB samples, the next block
-            // in flight while the current one is summed.
-            constexpr int kSB = 4;
-            int vtx      = -1;
-            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+            int cell = -1;  // (x0 + 1) + (G + 2) ((y0 + 1) + (G + 2) (z0 + 1)) of the current cell (x0 in [-1, G])
+            int cx0 = 0, cx1 = 0, cy0 = 0, cy1 = 0, cz0 = 0, cz1 = 0;
+            float acc[8][4];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.f;
             auto flush = [&]() {
-                if (vtx >= 0)
+                const int vz[2] = {cz0 * G * G, cz1 * G * G}, vy[2] = {cy0 * G, cy1 * G}, vx[2] = {cx0, cx1};
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
                 {
+                    const int v = vz[k >> 2] + vy[(k >> 1) & 1] + vx[k & 1];
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
-                        atomicAdd(sq + q * nv + vtx, (unsigned long long)__double2ll_rn((double)acc[q] * scale));
+                    {
+                        atomicAdd(sq + q * nv + v, (unsigned long long)__double2ll_rn((double)acc[k][q] * scale));
+                        acc[k][q] = 0.f;
+                    }
                 }
             };
-            auto load = [&](int64_t s0, float4 (&gb)[kSB], float4 (&db)[kSB]) {
+            constexpr int kSB = 4;
+            float4 ga[kSB], da[kSB];
+            auto load = [&](int64_t s0) {
 #pragma unroll
                 for (int b = 0; b < kSB; ++b)
                 {
                     if (s0 + b < e2)
                     {
-                        gb[b] = sgc[base + s0 + b];
-                        db[b] = dh[s0 + b];
+                        ga[b] = sgc[base + s0 + b];
+                        da[b] = dh[s0 + b];
                     }
                 }
             };
-            auto accumulate = [&](int64_t s0, const float4 (&gb)[kSB], const float4 (&db)[kSB]) {
+            load(s2);
+            for (int64_t s0 = s2; s0 < e2; s0 += kSB)
+            {
+                float4 gb[kSB], db[kSB];
+#pragma unroll
+                for (int b = 0; b < kSB; ++b)
+                {
+                    gb[b] = ga[b];
+                    db[b] = da[b];
+                }
+                load(s0 + kSB);  // the next block in flight while this one is summed
 #pragma unroll
                 for (int b = 0; b < kSB; ++b)
                 {
                     if (s0 + b >= e2) break;
                     const float4 gc = gb[b];
-                    // corner kk of corners_ix, the same operations
+                    // the trilinear corners of corners_ix, the same operations
                     const int x0 = (int)floorf(gc.x), y0 = (int)floorf(gc.y), z0 = (int)floorf(gc.z);
-                    const int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
-                    const float fx = (kk & 1) ? (gc.x - (float)x0) : ((float)x1 - gc.x);
-                    const float fy = (kk & 2) ? (gc.y - (float)y0) : ((float)y1 - gc.y);
-                    const float fz = (kk & 4) ? (gc.z - (float)z0) : ((float)z1 - gc.z);
-                    const float wk = fx * fy * fz;
-                    const int cx = clamp_g((kk & 1) ? x1 : x0, g.G), cy = clamp_g((kk & 2) ? y1 : y0, g.G),
-                              cz = clamp_g((kk & 4) ? z1 : z0, g.G);
-                    const int v = (cz * g.G + cy) * g.G + cx;
-                    if (v != vtx)
+                    const int c = (x0 + 1) + (G + 2) * ((y0 + 1) + (G + 2) * (z0 + 1));
+                    if (c != cell)
                     {
-                        flush();
-                        vtx    = v;
-                        acc[0] = acc[1] = acc[2] = acc[3] = 0.f;
+                        if (cell >= 0) flush();
+                        cell = c;
+                        cx0 = clamp_g(x0, G);
+                        cx1 = clamp_g(x0 + 1, G);
+                        cy0 = clamp_g(y0, G);
+                        cy1 = clamp_g(y0 + 1, G);
+                        cz0 = clamp_g(z0, G);
+                        cz1 = clamp_g(z0 + 1, G);
                     }
+                    const float fx1 = gc.x - (float)x0, fx0 = (float)(x0 + 1) - gc.x;
+                    const float fy1 = gc.y - (float)y0, fy0 = (float)(y0 + 1) - gc.y;
+                    const float fz1 = gc.z - (float)z0, fz0 = (float)(z0 + 1) - gc.z;
+                    const float wxy[4] = {fx0 * fy0, fx1 * fy0, fx0 * fy1, fx1 * fy1};
                     const float4 dv = db[b];
-                    acc[0] += wk * dv.x;
-                    acc[1] += wk * dv.y;
-                    acc[2] += wk * dv.z;
-                    acc[3] += wk * dv.w;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                    {
+                        // (fx * fy) * fz: the product order of corners_ix
+                        const float wk = wxy[k & 3] * ((k & 4) ? fz1 : fz0);
+                        acc[k][0] += wk * dv.x;
+                        acc[k][1] += wk * dv.y;
+                        acc[k][2] += wk * dv.z;
+                        acc[k][3] += wk * dv.w;
+                    }
                 }
-            };
-            float4 ga[kSB], gb[kSB], da[kSB], db[kSB];
-            load(s2, ga, da);
-            for (int64_t s0 = s2; s0 < e2; s0 += 2 * kSB)
-            {
-                load(s0 + kSB, gb, db);
-                accumulate(s0, ga, da);
-                load(s0 + 2 * kSB, ga, da);
-                accumulate(s0 + kSB, gb, db);
             }
-            flush();
+            if (cell >= 0) flush();
         }
         const int ch0 = 4 * (int)blockIdx.y;
         __syncthreads();
@@ -874,6 +924,7 @@ struct ProjSizes
     int64_t max1, max2;
     int32_t chunk2;
     size_t sort_temp;
+    bool prop;  // the segment list carries proposal CDF rows
 };
 
 int32_t scatter_chunk(int64_t S)
@@ -883,9 +934,9 @@ int32_t scatter_chunk(int64_t S)
     return (int32_t)std::max<int64_t>(2048, std::min<int64_t>(65536, t));
 }
 
-ProjSizes proj_sizes(int64_t K, int64_t S, int64_t N, int32_t G, int32_t C, int32_t P)
+ProjSizes proj_sizes(int64_t K, int64_t S, int64_t N, int32_t G, int32_t C, int32_t P, bool prop)
 {
-    ProjSizes z{K, S, N, G, C, P, 0, 0, scatter_chunk(S), 0};
+    ProjSizes z{K, S, N, G, C, P, 0, 0, scatter_chunk(S), 0, prop};
     z.max1 = (S + (int64_t)kSegCost * K + kFwdChunk - 1) / kFwdChunk + N;
     z.max2 = (S + z.chunk2 - 1) / z.chunk2 + N;
     size_t tmp = 0;
@@ -915,22 +966,31 @@ Layout proj_layout(const ProjSizes& z)
     L.add(4 * (size_t)z.S * z.C);         // 13 dfeat ([S][C]; matrix-core backward: [C/4][S][4])
     L.add(16 * (size_t)z.S);              // 14 grid coordinates (ix, iy, iz, -) per sample, matrix-core backward
     L.add(256);                           // 15 max |dL/dfeature| (float bits), matrix-core backward
+    L.add(4 * (kNodeWGMax + 1));          // 16 first chunk of each node-resident workgroup
+    L.add(64 * (size_t)z.K);              // 17 node-major segment records (ot, dt, i, h)
+    L.add(z.prop ? 32 * (size_t)z.K : 16);  // 18 node-major proposal cells / CDF rows
     return L;
+}
+
+// Node-resident workgroups: one per CU of the current device (at most kNodeWGMax).
+int node_workgroups()
+{
+    int dev = 0, cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                hipSuccess || cu <= 0)
+        return 256;
+    return std::min(cu, kNodeWGMax);
 }
 
 int bw1_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 2048)); }
 // matrix-core backward: persistent workgroups of 4 waves, one partial slice per wave
 int mfma_bwd_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 256)); }
-// k_mfma_bwd_relu: NEAT_RELU_BWD_WAVES workgroups per CU
-int mfma_bwd_relu_workgroups(const ProjSizes& z)
-{
-    return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 256 * NEAT_RELU_BWD_WAVES));
-}
 int mfma_fwd_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 1024)); }
 
 size_t partials_bytes(const ProjSizes& z)
 {
-    size_t slices = std::max<size_t>((size_t)bw1_workgroups(z), (size_t)mfma_bwd_relu_workgroups(z) * 4);
+    size_t slices = std::max<size_t>({(size_t)bw1_workgroups(z), (size_t)mfma_bwd_workgroups(z) * 4,
+                                      (size_t)kNodeWGMax * kNodeWaves});
     return (slices * z.P * 4 + 255) / 256 * 256;
 }
 
@@ -962,6 +1022,14 @@ Plan plan_view(void* ws, const ProjSizes& z)
     p.max1        = z.max1;
     p.max2        = z.max2;
     p.chunk2      = z.chunk2;
+    p.wg_first    = reinterpret_cast<int32_t*>(b + L.off[16]);
+    p.nwg         = node_workgroups();
+    char* rec     = b + L.off[17];
+    p.rec_ot      = reinterpret_cast<float4*>(rec);
+    p.rec_dt      = reinterpret_cast<float4*>(rec + 16 * z.K);
+    p.rec_i       = reinterpret_cast<int4*>(rec + 32 * z.K);
+    p.rec_h       = reinterpret_cast<uint4*>(rec + 48 * z.K);
+    p.rec_c       = z.prop ? reinterpret_cast<float4*>(b + L.off[18]) : nullptr;
     return p;
 }
 
@@ -981,7 +1049,8 @@ struct Args
     ProjSizes z;
     int precision;
     const void* grid_bf16;
-    bool mfma;  // matrix-core kernels (8 channels) rather than the vector-ALU ones
+    bool mfma;    // matrix-core kernels (8 channels) rather than the vector-ALU ones
+    bool relu16;  // BF16 relu decoder on the node-resident kernels (k_mfma_fwd_relu / k_mfma_bwd_relu)
 };
 
 int make_args(Args& a, const neat_projector_args* x)
@@ -1029,10 +1098,13 @@ int make_args(Args& a, const neat_projector_args* x)
     a.m         = MLP{x->in_features, x->hidden, x->hidden_layers, x->out_features};
     a.act       = x->act;
     a.params    = x->params;
-    a.z         = proj_sizes(K, S, N, G, C, a.m.nparams());
+    a.z         = proj_sizes(K, S, N, G, C, a.m.nparams(), x->seg_cdf != nullptr);
     a.precision = x->precision;
     a.grid_bf16 = x->grid_bf16;
     a.mfma      = x->precision != NEAT_PREC_FP32_VALU && C == 8;
+    // the node-resident relu kernels stage a node's BF16 grid in LDS (up to 17^3 vertices)
+    a.relu16    = a.mfma && x->precision == NEAT_PREC_BF16 && x->act == NEAT_ACT_RELU &&
+               (int64_t)G * G * G <= kStagedVerts;
     return NEAT_OK;
 }
 
@@ -1062,9 +1134,9 @@ template <typename P, int H>
 int launch_fwd_mfma(hipStream_t s, const Args& a, const Plan& p, float* seg_partial, float* sample_out = nullptr)
 {
     const unsigned grid = (unsigned)mfma_fwd_workgroups(a.z);
-    if (P::kBF16 && a.act == NEAT_ACT_RELU)
+    if (P::kBF16 && a.relu16)
     {
-        NEAT_REQUIRE(launch_mfma_fwd_relu(s, H, grid, a.g, p, a.params, a.grid_bf16, seg_partial, sample_out) == 0,
+        NEAT_REQUIRE(launch_mfma_fwd_relu(s, H, a.g, p, a.params, a.grid_bf16, seg_partial, sample_out) == 0,
                      "k_mfma_fwd_relu launch failed");
         return NEAT_OK;
     }
@@ -1136,9 +1208,9 @@ int launch_bw_valu(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b
 template <typename P, int H>
 int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b, int stage)
 {
-    // BF16 with relu hidden layers runs k_mfma_bwd_relu
-    const bool relu16 = P::kBF16 && a.act == NEAT_ACT_RELU;
-    const int nwg     = relu16 ? mfma_bwd_relu_workgroups(a.z) : mfma_bwd_workgroups(a.z);
+    // BF16 with relu hidden layers runs k_mfma_bwd_relu (node-resident: p.nwg workgroups of kNodeWaves waves)
+    const bool relu16 = P::kBF16 && a.relu16;
+    const int slices  = relu16 ? p.nwg * kNodeWaves : mfma_bwd_workgroups(a.z) * 4;
     const int Ptot    = a.m.nparams();
     if (stage == 1)
     {
@@ -1146,11 +1218,12 @@ int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b
         NEAT_LAUNCHED();
         if (relu16)
         {
-            NEAT_REQUIRE(launch_mfma_bwd_relu(s, H, (unsigned)nwg, a.g, p, a.params, a.grid_bf16, b.grad_proj, b.dfeat,
-                                              b.sgc, b.dmax, b.partials, Ptot) == 0,
+            NEAT_REQUIRE(launch_mfma_bwd_relu(s, H, a.g, p, a.params, a.grid_bf16, b.grad_proj, b.dfeat, b.sgc, b.dmax,
+                                              b.partials, Ptot) == 0,
                          "k_mfma_bwd_relu launch failed");
             return NEAT_OK;
         }
+        const int nwg = mfma_bwd_workgroups(a.z);
         if (a.act == NEAT_ACT_RELU)
             k_mfma_bwd<P, H, NEAT_ACT_RELU><<<nwg, 256, 0, s>>>(a.g, p, a.params, a.grid_bf16, b.grad_proj, b.dfeat,
                                                                  b.sgc, b.dmax, b.partials, Ptot);
@@ -1162,7 +1235,7 @@ int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b
     }
     if (stage == 2)
     {
-        k_dec_reduce_partials<<<reduce_partials_blocks(Ptot), 256, 0, s>>>(b.partials, nwg * 4, Ptot, b.grad_params);
+        k_dec_reduce_partials<<<reduce_partials_blocks(Ptot), 256, 0, s>>>(b.partials, slices, Ptot, b.grad_params);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }
@@ -1255,6 +1328,8 @@ extern "C"
             size_t tmp = z.sort_temp;
             NEAT_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tmp, keys, skeys, vals, p.perm, (int)z.K, 0, end_bit,
                                                         s));
+            k_plan_records<<<grid_for(z.K, 256), 256, 0, s>>>(a.g, p);
+            NEAT_LAUNCHED();
             k_plan_counts<<<grid_for(z.K, 256), 256, 0, s>>>(a.g, p.perm, p.nm_soff);
             NEAT_LAUNCHED();
         }
@@ -1273,6 +1348,8 @@ extern "C"
         if ((st = exclusive_scan_i64(s, p.chunk_off1, z.N, temp, temp_sz))) return st;
         if ((st = exclusive_scan_i64(s, p.chunk_off2, z.N, temp, temp_sz))) return st;
         k_plan_chunks<<<grid_for(z.max1 + z.max2, 256), 256, 0, s>>>(a.g, p);
+        NEAT_LAUNCHED();
+        k_plan_wg<<<grid_for(p.nwg + 1, 256), 256, 0, s>>>(a.g, p);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }

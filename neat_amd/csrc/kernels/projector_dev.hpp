@@ -15,6 +15,14 @@ constexpr int kSegCost    = 16;    // chunk cost of a segment, in samples
 constexpr int kSegCap     = kFwdChunk / kSegCost;  // segments per forward chunk, at most
 constexpr int kFwdThreads = 256;
 
+// Node-resident workgroups (the BF16 relu kernels): one workgroup of kNodeThreads per CU walks a contiguous,
+// cost-balanced run of forward chunks and keeps the current slot's BF16 node grid in LDS, so every corner gather
+// reads LDS and each node grid is fetched from HBM about once per pass.
+constexpr int kNodeThreads = 512;
+constexpr int kNodeWaves   = kNodeThreads / 64;
+constexpr int kNodeWGMax   = 512;   // workgroups a plan can describe
+constexpr int kStagedVerts = 4913;  // vertices of the largest staged node grid (17^3), 16 B each (8 BF16 channels)
+
 struct ProjGeom
 {
     const float* pmin;
@@ -56,6 +64,15 @@ struct Plan
     int32_t* chunk2_q1;
     int64_t max1, max2;
     int32_t chunk2;  // samples per scatter chunk
+    int32_t* wg_first;  // [nwg + 1] first forward chunk of each node-resident workgroup
+    int32_t nwg;        // node-resident workgroups the plan balanced the chunks over
+    // per-segment records in node-major order, written once by the plan so that a chunk's context is one coalesced
+    // read (the canonical arrays and the rays are read in scattered order there, once per plan)
+    float4* rec_ot;  // [K] (origin, tmin)
+    float4* rec_dt;  // [K] (direction, tmax)
+    int4* rec_i;     // [K] (canonical index, ray, rank in the ray, sample count)
+    uint4* rec_h;    // [K] (ray jitter hash low, high, kind, step bits); kind 0 uniform, 1 proposal cells, 2 CDF
+    float4* rec_c;   // [2K] kind 1: cell boundaries 1..7 (and 0); kind 2: the CDF row; null without proposal rows
 };
 
 // ---- per-sample geometry (bit-identical to the unfused sampler + ComputeLocalSamples chain) -----------------------
@@ -315,8 +332,8 @@ __device__ __forceinline__ const void* node_grid_ptr(const ProjGeom& g, const vo
                 : static_cast<const void*>(g.grid + off);
 }
 
-// Segment state of one forward chunk, staged in LDS once per chunk: what the chunk's samples need from the segment
-// list and the rays, so that a sample's geometry costs LDS reads rather than chains of dependent global loads.
+// Segment state of one forward chunk, staged in LDS once per chunk from the plan's node-major records: what the
+// chunk's samples need from the segment list and the rays, so that a sample's geometry costs LDS reads.
 struct ChunkCtx
 {
     int32_t beg[kSegCap + 1];  // chunk-relative first sample of each segment; beg[nseg] = the chunk's sample count
@@ -327,44 +344,85 @@ struct ChunkCtx
     float tmin[kSegCap], tmax[kSegCap], step[kSegCap];
     float gproj[kSegCap];  // dL/dprojection of the segment's ray (backward)
     float o[3][kSegCap], d[3][kSegCap];
-    float cdf[8][kSegCap];  // proposal CDF rows (cdf[7][t] == 0: uniform)
+    float cell[8][kSegCap];     // kind 1: proposal cell boundaries 1..7; kind 2: the proposal CDF row
     uint32_t hray[2][kSegCap];  // jitter_ray_hash of the segment's ray (low, high word)
-    uint16_t sq[kFwdChunk];  // window-relative sample -> chunk-relative segment
+    uint8_t kind[kSegCap];      // 0 uniform samples, 1 proposal cells precomputed (n <= 8), 2 proposal CDF
+    uint16_t sq[kFwdChunk];     // window-relative sample -> chunk-relative segment
 };
 
+// Record of the segment at node-major position q (k = perm[q]): what ChunkCtx holds, computed with the same
+// operations as sample_geo and proposal_cell so every derived value has the same bits.
+__device__ inline void make_record(const ProjGeom& g, const Plan& p, int64_t q)
+{
+    const int32_t k   = p.perm[q];
+    const int32_t ray = g.seg_ray[k];
+    const int32_t n   = g.seg_count[k];
+    const float tmin = g.seg_tmin[k], tmax = g.seg_tmax[k];
+    const float step  = (tmax - tmin) / (float)n;
+    const uint64_t hr = jitter_ray_hash(g.seed, ray);
+    uint32_t kind     = 0;
+    float cdf[8];
+    if (load_cdf(g, k, cdf))
+    {
+        kind = n <= 8 ? 1u : 2u;
+        float cell[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) cell[b] = cdf[b];
+        if (kind == 1)
+        {
+#pragma unroll
+            for (int j = 1; j <= 7; ++j) cell[j - 1] = proposal_cell(cdf, tmin, tmax, n, j);
+            cell[7] = 0.f;
+        }
+        p.rec_c[2 * q]     = make_float4(cell[0], cell[1], cell[2], cell[3]);
+        p.rec_c[2 * q + 1] = make_float4(cell[4], cell[5], cell[6], cell[7]);
+    }
+    p.rec_ot[q] = make_float4(g.ro[3 * (int64_t)ray], g.ro[3 * (int64_t)ray + 1], g.ro[3 * (int64_t)ray + 2], tmin);
+    p.rec_dt[q] = make_float4(g.rd[3 * (int64_t)ray], g.rd[3 * (int64_t)ray + 1], g.rd[3 * (int64_t)ray + 2], tmax);
+    p.rec_i[q]  = make_int4(k, ray, (int32_t)(k - g.ray_seg_offset[ray]), n);
+    p.rec_h[q]  = make_uint4((uint32_t)hr, (uint32_t)(hr >> 32), kind, __float_as_uint(step));
+}
+
 __device__ inline void load_chunk_ctx(const ProjGeom& g, const Plan& p, int32_t q0, int32_t q1, int64_t base,
-                               const float* grad_proj, ChunkCtx& c)
+                                      const float* grad_proj, ChunkCtx& c)
 {
     const int32_t nseg = q1 - q0;
     for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x)
     {
-        const int32_t q   = q0 + t;
-        const int32_t k   = p.perm[q];
-        const int32_t ray = g.seg_ray[k];
-        const int32_t n   = g.seg_count[k];
-        const float tmin = g.seg_tmin[k], tmax = g.seg_tmax[k];
+        const int32_t q = q0 + t;
+        const float4 ot = p.rec_ot[q], dt = p.rec_dt[q];
+        const int4 ri   = p.rec_i[q];
+        const uint4 rh  = p.rec_h[q];
         c.beg[t] = (int32_t)(p.nm_soff[q] - base);
         if (t == nseg - 1) c.beg[nseg] = (int32_t)(p.nm_soff[q1] - base);
-        c.k[t]     = k;
-        c.ray[t]   = ray;
-        c.rank[t]  = (int32_t)(k - g.ray_seg_offset[ray]);
-        c.n[t]     = n;
-        c.tmin[t]  = tmin;
-        c.tmax[t]  = tmax;
-        c.step[t]  = (tmax - tmin) / (float)n;
-        c.gproj[t] = grad_proj ? grad_proj[ray] : 0.f;
-        const uint64_t hr = jitter_ray_hash(g.seed, ray);
-        c.hray[0][t]      = (uint32_t)hr;
-        c.hray[1][t]      = (uint32_t)(hr >> 32);
-        float cdf[8];
-        const bool prop = load_cdf(g, k, cdf);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) c.cdf[b][t] = prop ? cdf[b] : 0.f;
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
+        c.k[t]       = ri.x;
+        c.ray[t]     = ri.y;
+        c.rank[t]    = ri.z;
+        c.n[t]       = ri.w;
+        c.tmin[t]    = ot.w;
+        c.tmax[t]    = dt.w;
+        c.step[t]    = __uint_as_float(rh.w);
+        c.gproj[t]   = grad_proj ? grad_proj[ri.y] : 0.f;
+        c.hray[0][t] = rh.x;
+        c.hray[1][t] = rh.y;
+        c.kind[t]    = (uint8_t)rh.z;
+        c.o[0][t] = ot.x;
+        c.o[1][t] = ot.y;
+        c.o[2][t] = ot.z;
+        c.d[0][t] = dt.x;
+        c.d[1][t] = dt.y;
+        c.d[2][t] = dt.z;
+        if (rh.z != 0)
         {
-            c.o[d][t] = g.ro[3 * (int64_t)ray + d];
-            c.d[d][t] = g.rd[3 * (int64_t)ray + d];
+            const float4 u = p.rec_c[2 * q], v = p.rec_c[2 * q + 1];
+            c.cell[0][t] = u.x;
+            c.cell[1][t] = u.y;
+            c.cell[2][t] = u.z;
+            c.cell[3][t] = u.w;
+            c.cell[4][t] = v.x;
+            c.cell[5][t] = v.y;
+            c.cell[6][t] = v.z;
+            c.cell[7][t] = v.w;
         }
     }
 }
@@ -397,13 +455,24 @@ __device__ __forceinline__ Sample ctx_sample(const ProjGeom& g, const ChunkCtx& 
     const uint64_t hr = ((uint64_t)c.hray[1][t] << 32) | c.hray[0][t];
     auto a_of         = [&](int32_t jj) { return g.jitter ? jitter_a_h(hr, rank, jj) : 0.5f; };
     float tt;
-    if (c.cdf[7][t] > 0.f)
+    const uint32_t kind = c.kind[t];
+    if (kind != 0)
     {
-        float cdf[8];
+        float t1, t2;
+        if (kind == 1)
+        {
+            // boundaries 1..7 precomputed by the plan (proposal_cell of the same CDF row)
+            t1 = j == 0 ? tmin : c.cell[j - 1][t];
+            t2 = j + 1 >= n ? tmax : c.cell[j][t];
+        }
+        else
+        {
+            float cdf[8];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) cdf[b] = c.cdf[b][t];
-        const float t1 = proposal_cell(cdf, tmin, tmax, n, j);
-        const float t2 = proposal_cell(cdf, tmin, tmax, n, j + 1);
+            for (int b = 0; b < 8; ++b) cdf[b] = c.cell[b][t];
+            t1 = proposal_cell(cdf, tmin, tmax, n, j);
+            t2 = proposal_cell(cdf, tmin, tmax, n, j + 1);
+        }
         const float a  = a_of(j);
         tt             = t1 * (1.0f - a) + t2 * a;
         s.w            = t2 - t1;
@@ -498,18 +567,35 @@ __device__ __forceinline__ void fetch_pair(const ProjGeom& g, const ChunkCtx& cx
     in.vb = __uint_as_float(vv[1]);
 }
 
-// ---- relu BF16 backward (projector_relu.hip) ----------------------------------------------------------------------
-// waves per SIMD the relu backward is compiled for (its register budget: 512 / waves per lane)
-#ifndef NEAT_RELU_BWD_WAVES
-#define NEAT_RELU_BWD_WAVES 2
-#endif
+// Copies a node's BF16 grid (nvert vertices x 8 channels, 16 B per vertex) into LDS, four loads in flight per thread.
+__device__ __forceinline__ void stage_node_grid(const uint4* __restrict__ src, uint4* dst, int32_t nvert)
+{
+    for (int32_t t0 = threadIdx.x; t0 < nvert; t0 += 4 * (int32_t)blockDim.x)
+    {
+        uint4 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            const int32_t t = t0 + k * (int32_t)blockDim.x;
+            if (t < nvert) v[k] = src[t];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            const int32_t t = t0 + k * (int32_t)blockDim.x;
+            if (t < nvert) dst[t] = v[k];
+        }
+    }
+}
 
-// Launches k_mfma_fwd_relu<H> (H = 32 or 64) on nwg persistent workgroups of 256 threads.
-int launch_mfma_fwd_relu(hipStream_t s, int H, unsigned nwg, const ProjGeom& g, const Plan& p, const float* params,
+// ---- relu BF16 backward (projector_relu.hip) ----------------------------------------------------------------------
+// Launches k_mfma_fwd_relu<H> (H = 32 or 64) on the plan's p.nwg node-resident workgroups of kNodeThreads.
+int launch_mfma_fwd_relu(hipStream_t s, int H, const ProjGeom& g, const Plan& p, const float* params,
                          const void* grid_bf16, float* seg_partial, float* sample_out);
 
-// Launches k_mfma_bwd_relu<H> (H = 32 or 64) on nwg persistent workgroups of 256 threads.
-int launch_mfma_bwd_relu(hipStream_t s, int H, unsigned nwg, const ProjGeom& g, const Plan& p, const float* params,
+// Launches k_mfma_bwd_relu<H> (H = 32 or 64) on the plan's p.nwg node-resident workgroups of kNodeThreads; every
+// wave writes one parameter-gradient slice (p.nwg * kNodeWaves slices).
+int launch_mfma_bwd_relu(hipStream_t s, int H, const ProjGeom& g, const Plan& p, const float* params,
                          const void* grid_bf16, const float* grad_proj, float* dfeat, float4* sgc, unsigned int* dmax,
                          float* partials, int32_t P_total);
 

@@ -6,15 +6,16 @@
 namespace neat
 {
 
-// Backward for BF16 relu decoders (the configuration the projector benchmarks): as k_mfma_bwd, but every product is
-// formed on the matrix cores in the orientation its consumer needs (WeightLDSRelu), so no tile is transposed
-// through LDS. Per wave and tile of 32 samples, with "h" tiles holding hidden units in the registers and samples on
-// the lanes, and "s" tiles the transpose:
-//   forward  Z1h, Z1s = W0 [x;1]; H1 = relu(Z1) (BF16); Z2h, Z2s = W1 H1 + b1; z = w2 . relu(Z2h) + b2
-//   backward dz = dL/dz; D2 = dz * [Z2 > 0] (h and s; the output weights are folded into w2t and the final scale)
+// Backward for BF16 relu decoders (the configuration the projector benchmarks). Per wave and tile of 32 samples,
+// with "h" fragments holding hidden units in the registers and samples on the lanes, and "s" fragments the transpose
+// (what the weight-gradient products, which contract over samples, take):
+//   forward  Z1h = W0 [x;1]; H1 = relu(Z1) (BF16); Z2h = W1 H1 + b1; z = w2 . relu(Z2h) + b2
+//   backward dz = dL/dz; D2 = dz * [Z2 > 0] (the output weights are folded into w2t and the final row scale)
 //            dW1 += D2s H1s^T; E += D2s [e10] + relu(Z2s) [dz e9] + D1s [x; 1]  (bias / output / first-layer grads)
-//            D1h, D1s = (w2t D2h) * [H1 > 0]; dL/dfeature = W0^T D1h
-// The relu masks come from the BF16 activations themselves, so only BF16 fragments stay live across the tile.
+//            D1h = (w2t D2h) * [H1 > 0]; dL/dfeature = W0^T D1h
+// Every product runs once, in h form; the s fragments of H1, relu(Z2) and D1 come from LDS transposes
+// (transpose_tile), and the s tile T of [x; 1; dz; 1] from one product against a 0/1 routing fragment. The relu
+// masks come from the BF16 activations themselves, so only BF16 fragments stay live across the tile.
 typedef short s16x8 __attribute__((ext_vector_type(8)));
 
 // Elementwise work on BF16 fragments runs as packed 16-bit integer operations on their bit patterns (two lanes of
@@ -67,17 +68,72 @@ __device__ __forceinline__ bf16x8 keep_active(const bf16x8& f, const bf16x8& h, 
 // relu for the FP32 output-layer dot product (one instruction; -0 may pass through, which adds nothing)
 __device__ __forceinline__ float relu_f32(float z) { return __builtin_amdgcn_fmed3f(z, 0.f, 3.402823466e38f); }
 
+// ---- fragment transposes through LDS ------------------------------------------------------------------------------
+// An "h" fragment (samples on the lanes: lane (n, g) holds rows 16st + 8(j/4) + 4g + j%4 of sample n for slot j) and
+// the "s" fragment of the same 32 x 32 tile (lane (m, g) holds samples crow(8st + j, g) of row m) are transposes of
+// each other. The h fragments of a tile go to a per-wave LDS tile [sample][row] (rows padded to 36 elements, 72 B,
+// so consecutive samples start on different banks) with two 8-byte stores each; the s fragments come back with
+// ds_read_b64_tr_b16, which hands lane 16G + i of a 16-lane group element e of the 8 bytes that lane 16G + 4e + i/4
+// points at, shifted by i % 4 elements: with lane 16G + 4e + q pointing at (sample 16st + 8r + 4(G/2) + e,
+// row 16(G%2) + 4q), lane (m, g) receives rows m of samples 16st + 8r + 4g + e, the s-fragment slots 4r + e.
+constexpr int kTrRow = 36;  // bf16 elements per sample row of a transpose tile
+
+typedef short s16x4_t __attribute__((ext_vector_type(4)));
+#define NEAT_LDS __attribute__((address_space(3)))
+
+__device__ __forceinline__ void tr_store(unsigned short* tile, const bf16x8& f, int st)
+{
+    const int l = threadIdx.x & 63, n = l & 31, g = l >> 5;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 v = __builtin_bit_cast(u32x4, f);
+    unsigned short* row = tile + n * kTrRow + 16 * st + 4 * g;
+    *reinterpret_cast<uint2*>(row)     = make_uint2(v[0], v[1]);
+    *reinterpret_cast<uint2*>(row + 8) = make_uint2(v[2], v[3]);
+}
+
+__device__ __forceinline__ bf16x8 tr_load(const unsigned short* tile, int st)
+{
+    const int l = threadIdx.x & 63, G = l >> 4, e = (l >> 2) & 3, q = l & 3;
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    u32x2 r[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+        const unsigned short* src = tile + (16 * st + 8 * h + 4 * (G >> 1) + e) * kTrRow + 16 * (G & 1) + 4 * q;
+        const s16x4_t v = __builtin_amdgcn_ds_read_tr16_b64_v4i16((NEAT_LDS s16x4_t*)(src));
+        r[h]            = __builtin_bit_cast(u32x2, v);
+    }
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 o = {r[0][0], r[0][1], r[1][0], r[1][1]};
+    return __builtin_bit_cast(bf16x8, o);
+}
+
+// s fragments (st = 0, 1) of the tile whose h fragments are hf, through this wave's transpose tile
+__device__ __forceinline__ void transpose_tile(unsigned short* tile, const bf16x8 (&hf)[2], bf16x8 (&sf)[2])
+{
+    wave_lds_fence();  // the previous reads of the tile are issued before these stores
+    tr_store(tile, hf[0], 0);
+    tr_store(tile, hf[1], 1);
+    wave_lds_fence();
+    sf[0] = tr_load(tile, 0);
+    sf[1] = tr_load(tile, 1);
+}
+
+// Node-resident: workgroup b walks forward chunks [wg_first[b], wg_first[b + 1]) and stages the BF16 grid of the
+// current slot in LDS whenever the slot changes (two waves per SIMD, one workgroup per CU).
 template <int H>
-__global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(ProjGeom g, Plan p, const float* __restrict__ params,
-                                                          const void* grid_lowp, const float* grad_proj, float* dfeat,
-                                                          float4* sgc, unsigned int* dmax, float* partials,
-                                                          int32_t P_total)
+__global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, Plan p, const float* __restrict__ params,
+                                                                   const void* grid_lowp, const float* grad_proj,
+                                                                   float* dfeat, float4* sgc, unsigned int* dmax,
+                                                                   float* partials, int32_t P_total)
 {
     constexpr int MT = H / 32, C = 8;
     typedef PrecBF16 P;
     __shared__ WeightLDSRelu<H> w;
     __shared__ ChunkCtx cx;
-    __shared__ __align__(16) float s_dz[4][32];
+    __shared__ __align__(16) float s_dz[kNodeWaves][32];
+    __shared__ uint4 s_grid[kStagedVerts];
+    __shared__ __align__(16) unsigned short s_tr[kNodeWaves][32 * kTrRow];
     load_weights_relu<H>(params, w);
     __syncthreads();  // the final partial write reads w.w3 even in a workgroup that gets no chunk
     const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
@@ -105,19 +161,26 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
     float b3 = 0.f;
     float amax = 0.f;
     float* sdz          = s_dz[wave];
+    unsigned short* trt = s_tr[wave];
     const uint32_t one2 = opaque_one2();
-    const int64_t nchunks = p.chunk_off1[g.N];
-    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x)
+    const int32_t nvert = g.G * g.G * g.G;
+    int32_t staged      = -1;
+    const int32_t cend  = p.wg_first[blockIdx.x + 1];
+    for (int32_t c = p.wg_first[blockIdx.x]; c < cend; ++c)
     {
         const int32_t slot = p.chunk1_slot[c];
         const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c], nseg = q1 - q0;
         if (nseg == 0) continue;
         const int64_t base = p.nm_soff[q0];
-        __syncthreads();
+        __syncthreads();  // the previous chunk's readers of cx and s_grid are done
+        if (slot != staged)
+        {
+            stage_node_grid(static_cast<const uint4*>(grid_lowp) + (int64_t)slot * nvert, s_grid, nvert);
+            staged = slot;
+        }
         load_chunk_ctx(g, p, q0, q1, base, grad_proj, cx);
         NodeBox nb;
         node_bounds(g, g.seg_node[p.perm[q0]], nb);
-        const void* ngrid = node_grid_ptr(g, grid_lowp, true, slot);
         __syncthreads();
         const int32_t nsamp = cx.beg[nseg];
         for (int32_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
@@ -136,25 +199,19 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
                     X[4] = (__bf16)1.f;
                     X[5] = (__bf16)1.f;
                 }
-                // first layer, both orientations
-                bf16x8 H1h[MT][2], H1s[MT][2];
+                // first layer (h)
+                bf16x8 H1h[MT][2];
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
                 {
                     const f32x16 zh = P::mma(w.w1e[mt][l], X, f32x16{});
-                    const f32x16 zs = P::mma(X, w.w1e[mt][l], f32x16{});
 #pragma unroll
-                    for (int st = 0; st < 2; ++st)
-                    {
-                        H1h[mt][st] = relu_bf16(bf16_frag(zh, st));
-                        H1s[mt][st] = relu_bf16(bf16_frag(zs, st));
-                    }
+                    for (int st = 0; st < 2; ++st) H1h[mt][st] = relu_bf16(bf16_frag(zh, st));
                 }
-                // hidden layer (h): output pre-activation and dz
-                bf16x8 D2h[MT][2];
+                // hidden layer (h): output pre-activation, dz, relu(Z2) and D2 = dz [Z2 > 0]
+                bf16x8 H2h[MT][2], D2h[MT][2];
                 float dz;
                 {
-                    f32x16 Z2h[MT];
                     float part = 0.f;
 #pragma unroll
                     for (int mt = 0; mt < MT; ++mt)
@@ -166,7 +223,6 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
 #pragma unroll
                             for (int st = 0; st < 2; ++st) acc = P::mma(w.w2[mt][ms][st][l], H1h[ms][st], acc);
                         }
-                        Z2h[mt] = acc;
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
                         {
@@ -176,6 +232,8 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
                             part = fmaf(w3.z, relu_f32(acc[4 * q + 2]), part);
                             part = fmaf(w3.w, relu_f32(acc[4 * q + 3]), part);
                         }
+#pragma unroll
+                        for (int st = 0; st < 2; ++st) H2h[mt][st] = relu_bf16(bf16_frag(acc, st));
                     }
                     const float z = part + __shfl_xor(part, 32) + w.b3;
                     dz = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, z, g.beta) : 0.f;
@@ -186,7 +244,7 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
                     for (int mt = 0; mt < MT; ++mt)
                     {
 #pragma unroll
-                        for (int st = 0; st < 2; ++st) D2h[mt][st] = keep_active(dz8, relu_bf16(bf16_frag(Z2h[mt], st)), one2);
+                        for (int st = 0; st < 2; ++st) D2h[mt][st] = keep_active(dz8, H2h[mt][st], one2);
                     }
                 }
                 if (hg == 0) b3 += dz;
@@ -212,50 +270,34 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
 #pragma unroll
                     for (int st = 0; st < 2; ++st)
                     {
-                        bf16x8 t;
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) t[j] = (__bf16)T[8 * st + j];
+                        const bf16x8 t = bf16_frag(T, st);
                         B08[st] = n <= 8 ? t : bf16x8{};
                         B9[st]  = n == 9 ? t : bf16x8{};
                         B10[st] = n == 10 ? t : bf16x8{};
                     }
                 }
-                // hidden layer (s): relu(Z2s) and D2s; output-weight and hidden-bias gradients; hidden-weight gradient
+                // s fragments of H1 (hidden-weight gradient) by transposes
+                bf16x8 H1s[MT][2];
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) transpose_tile(trt, H1h[mt], H1s[mt]);
+                // relu(Z2) and D2 in s form; output-weight and hidden-bias gradients; hidden-weight gradient
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
                 {
-                    f32x16 acc = P::mma(ones, w.w2b[mt][l], f32x16{});
-#pragma unroll
-                    for (int ms = 0; ms < MT; ++ms)
-                    {
-#pragma unroll
-                        for (int st = 0; st < 2; ++st) acc = P::mma(H1h[ms][st], w.w2[mt][ms][st][l], acc);
-                    }
+                    bf16x8 H2s[2];
+                    transpose_tile(trt, H2h[mt], H2s);
 #pragma unroll
                     for (int st = 0; st < 2; ++st)
                     {
-                        const bf16x8 h2 = relu_bf16(bf16_frag(acc, st)), d2 = keep_active(dzs[st], h2, one2);
-                        E[mt] = P::mma(h2, B9[st], E[mt]);
+                        const bf16x8 d2 = keep_active(dzs[st], H2s[st], one2);
+                        E[mt] = P::mma(H2s[st], B9[st], E[mt]);
                         E[mt] = P::mma(d2, B10[st], E[mt]);
 #pragma unroll
                         for (int m1 = 0; m1 < MT; ++m1) dW2[mt][m1] = P::mma(d2, H1s[m1][st], dW2[mt][m1]);
                     }
                 }
-                // first layer (s): D1s and the first-layer weight / bias gradients
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                {
-                    f32x16 acc = {};
-#pragma unroll
-                    for (int ms = 0; ms < MT; ++ms)
-                    {
-#pragma unroll
-                        for (int st = 0; st < 2; ++st) acc = P::mma(D2h[ms][st], w.w2t[mt][ms][st][l], acc);
-                    }
-#pragma unroll
-                    for (int st = 0; st < 2; ++st) E[mt] = P::mma(keep_active(bf16_frag(acc, st), H1s[mt][st], one2), B08[st], E[mt]);
-                }
-                // first layer (h): D1h and dL/dfeature = W0^T D1h (rows 0..7: registers 0..3 of each lane group)
+                // first layer: D1 = (w2t D2) [H1 > 0] in h form, dL/dfeature = W0^T D1, and in s form the first-layer
+                // weight / bias gradients
                 f32x16 dx = {};
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
@@ -267,8 +309,16 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
 #pragma unroll
                         for (int st = 0; st < 2; ++st) acc = P::mma(w.w2t[mt][ms][st][l], D2h[ms][st], acc);
                     }
+                    bf16x8 D1h[2], D1s[2];
 #pragma unroll
-                    for (int st = 0; st < 2; ++st) dx = P::mma(w.w1t[mt][st][l], keep_active(bf16_frag(acc, st), H1h[mt][st], one2), dx);
+                    for (int st = 0; st < 2; ++st)
+                    {
+                        D1h[st] = keep_active(bf16_frag(acc, st), H1h[mt][st], one2);
+                        dx      = P::mma(w.w1t[mt][st][l], D1h[st], dx);
+                    }
+                    transpose_tile(trt, D1h, D1s);
+#pragma unroll
+                    for (int st = 0; st < 2; ++st) E[mt] = P::mma(D1s[st], B08[st], E[mt]);
                 }
                 if (valid)
                 {
@@ -277,10 +327,10 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
                     amax = fmaxf(amax, fmaxf(fmaxf(fabsf(dx[0]), fabsf(dx[1])), fmaxf(fabsf(dx[2]), fabsf(dx[3]))));
                 }
             };
-            for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
+            for (int32_t p0 = wave * 64; p0 < wn; p0 += kNodeThreads)
             {
                 PairIn cur;
-                fetch_pair<true, true>(g, cx, p0, w0, wn, base, nb, ngrid, sgc, cur);
+                fetch_pair<true, true>(g, cx, p0, w0, wn, base, nb, s_grid, sgc, cur);
 #pragma unroll 1
                 for (int h = 0; h < 2; ++h)
                 {
@@ -296,7 +346,7 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
     }
     // ---- this wave's slice of the partial parameter gradient (packed layout); rows scaled by the output weights
     // where the output weights were folded out of D2 ----
-    float* part   = partials + ((int64_t)blockIdx.x * 4 + wave) * P_total;
+    float* part   = partials + ((int64_t)blockIdx.x * kNodeWaves + wave) * P_total;
     const int oW0 = 0, ob0 = H * C, oW1 = ob0 + H, ob1 = oW1 + H * H, oW2 = ob1 + H, ob2 = oW2 + H;
 #pragma unroll
     for (int m2 = 0; m2 < MT; ++m2)
@@ -325,9 +375,11 @@ __global__ void __launch_bounds__(256, NEAT_RELU_BWD_WAVES) k_mfma_bwd_relu(Proj
 // Forward for BF16 relu decoders: k_mfma_fwd's structure (persistent workgroups over the chunks, tiles of 32
 // samples per wave, per-segment sums sequential in j) with the first- and hidden-layer biases in the products'
 // spare k slots and the elementwise work on packed BF16 fragments.
+// Node-resident like the backward.
 template <int H>
-__global__ void __launch_bounds__(256, 2) k_mfma_fwd_relu(ProjGeom g, Plan p, const float* __restrict__ params,
-                                                          const void* grid_lowp, float* seg_partial, float* sample_out)
+__global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_fwd_relu(ProjGeom g, Plan p, const float* __restrict__ params,
+                                                                   const void* grid_lowp, float* seg_partial,
+                                                                   float* sample_out)
 {
     constexpr int MT = H / 32;
     typedef PrecBF16 P;
@@ -335,6 +387,7 @@ __global__ void __launch_bounds__(256, 2) k_mfma_fwd_relu(ProjGeom g, Plan p, co
     __shared__ ChunkCtx cx;
     __shared__ float s_contrib[kFwdChunk];
     __shared__ float s_seg[kSegCap];
+    __shared__ uint4 s_grid[kStagedVerts];
     load_weights_relu<H>(params, w);
     const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
     bf16x8 ones = {};
@@ -343,19 +396,25 @@ __global__ void __launch_bounds__(256, 2) k_mfma_fwd_relu(ProjGeom g, Plan p, co
         ones[0] = (__bf16)1.f;
         ones[1] = (__bf16)1.f;
     }
-    const int64_t nchunks = p.chunk_off1[g.N];
-    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x)
+    const int32_t nvert = g.G * g.G * g.G;
+    int32_t staged      = -1;
+    const int32_t cend  = p.wg_first[blockIdx.x + 1];
+    for (int32_t c = p.wg_first[blockIdx.x]; c < cend; ++c)
     {
         const int32_t slot = p.chunk1_slot[c];
         const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c], nseg = q1 - q0;
         if (nseg == 0) continue;
         const int64_t base = p.nm_soff[q0];
-        __syncthreads();  // the previous chunk's readers of cx / s_seg are done (and the weights are in place)
+        __syncthreads();  // the previous chunk's readers of cx / s_seg / s_grid are done (and the weights are in place)
+        if (slot != staged)
+        {
+            stage_node_grid(static_cast<const uint4*>(grid_lowp) + (int64_t)slot * nvert, s_grid, nvert);
+            staged = slot;
+        }
         load_chunk_ctx(g, p, q0, q1, base, nullptr, cx);
         for (int32_t t = threadIdx.x; t < nseg; t += blockDim.x) s_seg[t] = 0.f;
         NodeBox nb;
         node_bounds(g, g.seg_node[p.perm[q0]], nb);
-        const void* ngrid = node_grid_ptr(g, grid_lowp, true, slot);
         __syncthreads();
         const int32_t nsamp = cx.beg[nseg];
         for (int32_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
@@ -409,10 +468,10 @@ __global__ void __launch_bounds__(256, 2) k_mfma_fwd_relu(ProjGeom g, Plan p, co
                     if (sample_out) sample_out[base + w0 + i] = y;
                 }
             };
-            for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
+            for (int32_t p0 = wave * 64; p0 < wn; p0 += kNodeThreads)
             {
                 PairIn cur;
-                fetch_pair<true, false>(g, cx, p0, w0, wn, base, nb, ngrid, nullptr, cur);
+                fetch_pair<true, false>(g, cx, p0, w0, wn, base, nb, s_grid, nullptr, cur);
                 tile(p0 + n, cur.xa, cur.va);
                 if (p0 + 32 < wn) tile(p0 + 32 + n, cur.xb, cur.vb);
             }
@@ -431,24 +490,26 @@ __global__ void __launch_bounds__(256, 2) k_mfma_fwd_relu(ProjGeom g, Plan p, co
     }
 }
 
-int launch_mfma_fwd_relu(hipStream_t s, int H, unsigned nwg, const ProjGeom& g, const Plan& p, const float* params,
+int launch_mfma_fwd_relu(hipStream_t s, int H, const ProjGeom& g, const Plan& p, const float* params,
                          const void* grid_bf16, float* seg_partial, float* sample_out)
 {
     if (H == 64)
-        k_mfma_fwd_relu<64><<<nwg, 256, 0, s>>>(g, p, params, grid_bf16, seg_partial, sample_out);
+        k_mfma_fwd_relu<64><<<p.nwg, kNodeThreads, 0, s>>>(g, p, params, grid_bf16, seg_partial, sample_out);
     else
-        k_mfma_fwd_relu<32><<<nwg, 256, 0, s>>>(g, p, params, grid_bf16, seg_partial, sample_out);
+        k_mfma_fwd_relu<32><<<p.nwg, kNodeThreads, 0, s>>>(g, p, params, grid_bf16, seg_partial, sample_out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_mfma_bwd_relu(hipStream_t s, int H, unsigned nwg, const ProjGeom& g, const Plan& p, const float* params,
+int launch_mfma_bwd_relu(hipStream_t s, int H, const ProjGeom& g, const Plan& p, const float* params,
                          const void* grid_bf16, const float* grad_proj, float* dfeat, float4* sgc, unsigned int* dmax,
                          float* partials, int32_t P_total)
 {
     if (H == 64)
-        k_mfma_bwd_relu<64><<<nwg, 256, 0, s>>>(g, p, params, grid_bf16, grad_proj, dfeat, sgc, dmax, partials, P_total);
+        k_mfma_bwd_relu<64><<<p.nwg, kNodeThreads, 0, s>>>(g, p, params, grid_bf16, grad_proj, dfeat, sgc, dmax,
+                                                            partials, P_total);
     else
-        k_mfma_bwd_relu<32><<<nwg, 256, 0, s>>>(g, p, params, grid_bf16, grad_proj, dfeat, sgc, dmax, partials, P_total);
+        k_mfma_bwd_relu<32><<<p.nwg, kNodeThreads, 0, s>>>(g, p, params, grid_bf16, grad_proj, dfeat, sgc, dmax,
+                                                            partials, P_total);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

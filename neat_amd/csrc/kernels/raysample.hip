@@ -10,6 +10,8 @@
 //
 // Output order is canonical (segments of a ray sorted by (tmin, node id)); the whole file is compiled without
 // floating-point contraction so counts and t values are bit-identical to the oracle.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace neat
@@ -173,6 +175,59 @@ __device__ __forceinline__ bool seg_less(float ta, int32_t na, float tb, int32_t
     return ta < tb || (ta == tb && na < nb);
 }
 
+// Segment list of one ray in canonical order into columns at stride `st` from position `base` (at most `cap`
+// entries; `count` and `samples` receive the ray's full segment and sample counts): insertion into the sorted
+// prefix while walking (segments per ray are few, and mostly arrive in order, which the register copy of the last
+// entry answers without a memory read).
+__device__ __forceinline__ void walk_sorted(const TreeArgs& T, const float o[3], const float d[3], int32_t max_spn,
+                                            int32_t* node_col, int32_t* count_col, float* tmin_col, float* tmax_col,
+                                            int64_t base, int64_t st, int64_t cap, int64_t& count, int64_t& samples)
+{
+    int64_t k      = 0;  // entries written
+    int64_t nk     = 0;  // segments seen
+    int64_t ns     = 0;
+    float last_t   = 0.f;  // the largest (tmin, node) so far, i.e. the entry at k - 1 (valid when k > 0)
+    int32_t last_n = 0;
+    auto at = [&](int64_t i) { return base + i * st; };
+    traverse(T, o, d,
+             [&](int32_t node, float tmin, float tmax, const float* mn, const float* mx)
+             {
+                 const int32_t cnt = segment_count(mn, mx, tmin, tmax, max_spn);
+                 nk += 1;
+                 ns += cnt;
+                 if (k >= cap) return;
+                 int64_t p = k;
+                 if (k == 0 || !seg_less(tmin, node, last_t, last_n))
+                 {
+                     last_t = tmin;
+                     last_n = node;
+                 }
+                 else
+                 {
+                     tmin_col[at(p)]  = last_t;
+                     tmax_col[at(p)]  = tmax_col[at(p - 1)];
+                     node_col[at(p)]  = last_n;
+                     count_col[at(p)] = count_col[at(p - 1)];
+                     --p;
+                 }
+                 while (p > 0 && p < k && seg_less(tmin, node, tmin_col[at(p - 1)], node_col[at(p - 1)]))
+                 {
+                     tmin_col[at(p)]  = tmin_col[at(p - 1)];
+                     tmax_col[at(p)]  = tmax_col[at(p - 1)];
+                     node_col[at(p)]  = node_col[at(p - 1)];
+                     count_col[at(p)] = count_col[at(p - 1)];
+                     --p;
+                 }
+                 tmin_col[at(p)]  = tmin;
+                 tmax_col[at(p)]  = tmax;
+                 node_col[at(p)]  = node;
+                 count_col[at(p)] = cnt;
+                 ++k;
+             });
+    count   = nk;
+    samples = ns;
+}
+
 __global__ void __launch_bounds__(128) k_ray_segments(TreeArgs T, const float* ro, const float* rd, int64_t R,
                                                       int32_t max_spn, const int64_t* ray_seg_offset,
                                                       const int64_t* ray_sample_offset, int32_t* seg_node,
@@ -185,44 +240,74 @@ __global__ void __launch_bounds__(128) k_ray_segments(TreeArgs T, const float* r
     load_ray(ro, rd, r, o, d);
     const int64_t base = ray_seg_offset[r];
     const int64_t end  = ray_seg_offset[r + 1];
-    int64_t k          = base;
-    float last_t       = 0.f;  // the largest (tmin, node) so far, i.e. the entry at k - 1 (valid when k > base)
-    int32_t last_n     = 0;
-    // insertion into the sorted prefix [base, k) while walking (segments per ray are few, and mostly arrive in
-    // order, which the register copy of the last entry answers without a memory read)
-    traverse(T, o, d,
-             [&](int32_t node, float tmin, float tmax, const float* mn, const float* mx)
-             {
-                 if (k >= end) return;  // cannot happen when the offsets come from k_ray_count
-                 int32_t cnt = segment_count(mn, mx, tmin, tmax, max_spn);
-                 int64_t p   = k;
-                 if (k == base || !seg_less(tmin, node, last_t, last_n))
-                 {
-                     last_t = tmin;
-                     last_n = node;
-                 }
-                 else
-                 {
-                     seg_tmin[p]  = last_t;
-                     seg_tmax[p]  = seg_tmax[p - 1];
-                     seg_node[p]  = last_n;
-                     seg_count[p] = seg_count[p - 1];
-                     --p;
-                 }
-                 while (p > base && p < k && seg_less(tmin, node, seg_tmin[p - 1], seg_node[p - 1]))
-                 {
-                     seg_tmin[p]  = seg_tmin[p - 1];
-                     seg_tmax[p]  = seg_tmax[p - 1];
-                     seg_node[p]  = seg_node[p - 1];
-                     seg_count[p] = seg_count[p - 1];
-                     --p;
-                 }
-                 seg_tmin[p]  = tmin;
-                 seg_tmax[p]  = tmax;
-                 seg_node[p]  = node;
-                 seg_count[p] = cnt;
-                 ++k;
-             });
+    int64_t nk, ns;
+    // the capacity end - base holds every segment when the offsets come from the same walk
+    walk_sorted(T, o, d, max_spn, seg_node, seg_count, seg_tmin, seg_tmax, base, 1, end - base, nk, ns);
+    int64_t so = ray_sample_offset[r];
+    for (int64_t q = base; q < end; ++q)
+    {
+        seg_ray[q]           = (int32_t)r;
+        seg_sample_offset[q] = so;
+        so += seg_count[q];
+    }
+}
+
+// Single-walk sampler: segment i of ray r goes to column entry i * R + r of a scratch of `cap` entries per ray
+// (four columns: node, count, tmin, tmax), with the ray's segment and sample counts; k_ray_compact then copies the
+// scratch into the canonical list once the offsets are known (a ray with more than `cap` segments is walked again
+// there).
+struct RayScratch
+{
+    int32_t* node;
+    int32_t* count;
+    float* tmin;
+    float* tmax;
+    int64_t cap;
+};
+
+__global__ void __launch_bounds__(128) k_ray_walk(TreeArgs T, const float* ro, const float* rd, int64_t R,
+                                                  int32_t max_spn, RayScratch sc, int64_t* nseg, int64_t* nsamp,
+                                                  int32_t* max_per_ray)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    float o[3], d[3];
+    load_ray(ro, rd, r, o, d);
+    int64_t nk, ns;
+    walk_sorted(T, o, d, max_spn, sc.node, sc.count, sc.tmin, sc.tmax, r, R, sc.cap, nk, ns);
+    nseg[r]  = nk;
+    nsamp[r] = ns;
+    if (max_per_ray) atomicMax(max_per_ray, (int32_t)ns);
+}
+
+__global__ void __launch_bounds__(128) k_ray_compact(TreeArgs T, const float* ro, const float* rd, int64_t R,
+                                                     int32_t max_spn, RayScratch sc, const int64_t* ray_seg_offset,
+                                                     const int64_t* ray_sample_offset, int32_t* seg_node,
+                                                     int32_t* seg_ray, int32_t* seg_count, float* seg_tmin,
+                                                     float* seg_tmax, int64_t* seg_sample_offset)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const int64_t base = ray_seg_offset[r];
+    const int64_t end  = ray_seg_offset[r + 1];
+    if (end - base > sc.cap)
+    {
+        float o[3], d[3];
+        load_ray(ro, rd, r, o, d);
+        int64_t nk, ns;
+        walk_sorted(T, o, d, max_spn, seg_node, seg_count, seg_tmin, seg_tmax, base, 1, end - base, nk, ns);
+    }
+    else
+    {
+        for (int64_t i = 0; i < end - base; ++i)
+        {
+            const int64_t e    = i * R + r;
+            seg_node[base + i]  = sc.node[e];
+            seg_count[base + i] = sc.count[e];
+            seg_tmin[base + i]  = sc.tmin[e];
+            seg_tmax[base + i]  = sc.tmax[e];
+        }
+    }
     int64_t so = ray_sample_offset[r];
     for (int64_t q = base; q < end; ++q)
     {
@@ -289,9 +374,91 @@ static int check_tree(const float* pmin, const float* pmax, const uint8_t* flags
     return NEAT_OK;
 }
 
+// Scratch entries per ray of the single-walk sampler: the most leaves a ray can cross in a depth-D octree with
+// positive length (3 * 2^D - 2, one per crossed cell plane, plus margin), at most 128.
+static int64_t scratch_cap(int32_t max_depth)
+{
+    const int64_t c = 3 * ((int64_t)1 << std::min(max_depth, 9)) + 8;
+    return std::min<int64_t>(c, 128);
+}
+
+static RayScratch scratch_view(void* p, int64_t R, int32_t max_depth)
+{
+    RayScratch sc;
+    sc.cap        = scratch_cap(max_depth);
+    const size_t n = (size_t)(sc.cap * R);
+    char* b       = static_cast<char*>(p);
+    sc.node       = reinterpret_cast<int32_t*>(b);
+    sc.count      = reinterpret_cast<int32_t*>(b + 4 * n);
+    sc.tmin       = reinterpret_cast<float*>(b + 8 * n);
+    sc.tmax       = reinterpret_cast<float*>(b + 12 * n);
+    return sc;
+}
+
 extern "C"
 {
     size_t neat_raysample_workspace_bytes(int64_t num_rays) { return scan_workspace_bytes(num_rays); }
+
+    size_t neat_raysample_scratch_bytes(int64_t num_rays, int32_t max_depth)
+    {
+        if (num_rays < 0 || max_depth < 0 || max_depth > 9) return 0;
+        return (size_t)(16 * scratch_cap(max_depth) * std::max<int64_t>(num_rays, 1));
+    }
+
+    int neat_raysample_walk(neat_stream_t stream, const float* pmin, const float* pmax, const uint8_t* flags,
+                            int32_t num_nodes, int32_t max_depth, const float* ro, const float* rd, int64_t R,
+                            int32_t max_spn, int64_t* ray_seg_offset, int64_t* ray_sample_offset,
+                            int32_t* max_per_ray, void* scratch, size_t scratch_bytes, void* workspace,
+                            size_t workspace_bytes)
+    {
+        int st = check_tree(pmin, pmax, flags, num_nodes, max_depth);
+        if (st) return st;
+        NEAT_REQUIRE(R >= 0, "num_rays %lld < 0", (long long)R);
+        NEAT_REQUIRE(max_spn > 0, "max_samples_per_node %d must be > 0", max_spn);
+        NEAT_REQUIRE(ray_seg_offset && ray_sample_offset, "offset arrays must not be null");
+        NEAT_REQUIRE(R == 0 || (ro && rd), "ray arrays must not be null");
+        NEAT_REQUIRE(R == 0 || (scratch && scratch_bytes >= neat_raysample_scratch_bytes(R, max_depth)),
+                     "neat_raysample_walk: scratch too small");
+        NEAT_REQUIRE((reinterpret_cast<uintptr_t>(scratch) % 16) == 0, "neat_raysample_walk: scratch alignment");
+        hipStream_t s = as_stream(stream);
+        k_zero_i32<<<1, 1, 0, s>>>(max_per_ray);
+        NEAT_LAUNCHED();
+        if (R > 0)
+        {
+            TreeArgs T{pmin, pmax, flags, first_leaf(num_nodes)};
+            k_ray_walk<<<grid_for(R, 128), 128, 0, s>>>(T, ro, rd, R, max_spn, scratch_view(scratch, R, max_depth),
+                                                         ray_seg_offset, ray_sample_offset, max_per_ray);
+            NEAT_LAUNCHED();
+        }
+        st = exclusive_scan_i64(s, ray_seg_offset, R, workspace, workspace_bytes);
+        if (st) return st;
+        return exclusive_scan_i64(s, ray_sample_offset, R, workspace, workspace_bytes);
+    }
+
+    int neat_raysample_compact(neat_stream_t stream, const float* pmin, const float* pmax, const uint8_t* flags,
+                               int32_t num_nodes, int32_t max_depth, const float* ro, const float* rd, int64_t R,
+                               int32_t max_spn, const int64_t* ray_seg_offset, const int64_t* ray_sample_offset,
+                               const void* scratch, size_t scratch_bytes, int64_t K, int32_t* seg_node,
+                               int32_t* seg_ray, int32_t* seg_count, float* seg_tmin, float* seg_tmax,
+                               int64_t* seg_sample_offset)
+    {
+        int st = check_tree(pmin, pmax, flags, num_nodes, max_depth);
+        if (st) return st;
+        NEAT_REQUIRE(R >= 0 && K >= 0, "negative size");
+        NEAT_REQUIRE(max_spn > 0, "max_samples_per_node %d must be > 0", max_spn);
+        if (R == 0 || K == 0) return NEAT_OK;
+        NEAT_REQUIRE(ro && rd && ray_seg_offset && ray_sample_offset && seg_node && seg_ray && seg_count && seg_tmin &&
+                         seg_tmax && seg_sample_offset,
+                     "neat_raysample_compact: null pointer");
+        NEAT_REQUIRE(scratch && scratch_bytes >= neat_raysample_scratch_bytes(R, max_depth),
+                     "neat_raysample_compact: scratch too small");
+        TreeArgs T{pmin, pmax, flags, first_leaf(num_nodes)};
+        k_ray_compact<<<grid_for(R, 128), 128, 0, as_stream(stream)>>>(
+            T, ro, rd, R, max_spn, scratch_view(const_cast<void*>(scratch), R, max_depth), ray_seg_offset,
+            ray_sample_offset, seg_node, seg_ray, seg_count, seg_tmin, seg_tmax, seg_sample_offset);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
 
     int neat_raysample_count(neat_stream_t stream, const float* pmin, const float* pmax, const uint8_t* flags,
                              int32_t num_nodes, int32_t max_depth, const float* ro, const float* rd, int64_t R,

@@ -39,8 +39,9 @@ def tree_validate(pos_min, pos_max, children, max_depth):
 
 
 def raysample(pos_min, pos_max, flags, max_depth, origin, direction, max_spn, jitter=False, seed=0, emit=True,
-              stream=None):
-    """Canonical segment list (+ SampleList columns when emit) for rays [R,3] on the GPU."""
+              stream=None, single_walk=True):
+    """Canonical segment list (+ SampleList columns when emit) for rays [R,3] on the GPU: one walk per ray into a
+    per-ray scratch and a compaction (single_walk), or the count walk and a second walk that writes the list."""
     dev = origin.device
     s = stream_ptr(stream)
     R = origin.shape[0]
@@ -52,17 +53,28 @@ def raysample(pos_min, pos_max, flags, max_depth, origin, direction, max_spn, ji
     maxr = torch.empty(1, dtype=_I32, device=dev)
     ws_bytes = _abi.lib().neat_raysample_workspace_bytes(R)
     ws = torch.empty(max(ws_bytes, 8), dtype=torch.uint8, device=dev)
-    call("neat_raysample_count", s, ptr(pos_min), ptr(pos_max), ptr(flags), num_nodes, max_depth, ptr(o), ptr(d), R,
-         max_spn, ptr(seg_off), ptr(samp_off), ptr(maxr), ptr(ws), ws_bytes)
-    K = int(seg_off[R].item())
-    S = int(samp_off[R].item())
-    r = dict(ray_seg_offset=seg_off, ray_sample_offset=samp_off, max_samples_per_ray=int(maxr.item()),
+    if single_walk:
+        sc_bytes = _abi.lib().neat_raysample_scratch_bytes(R, max_depth)
+        scratch = torch.empty(max(sc_bytes, 16), dtype=torch.uint8, device=dev)
+        call("neat_raysample_walk", s, ptr(pos_min), ptr(pos_max), ptr(flags), num_nodes, max_depth, ptr(o), ptr(d),
+             R, max_spn, ptr(seg_off), ptr(samp_off), ptr(maxr), ptr(scratch), sc_bytes, ptr(ws), ws_bytes)
+    else:
+        call("neat_raysample_count", s, ptr(pos_min), ptr(pos_max), ptr(flags), num_nodes, max_depth, ptr(o), ptr(d),
+             R, max_spn, ptr(seg_off), ptr(samp_off), ptr(maxr), ptr(ws), ws_bytes)
+    tot = torch.stack([seg_off[R], samp_off[R], maxr[0].to(_I64)]).cpu()  # one device -> host read
+    K, S, maxs = int(tot[0]), int(tot[1]), int(tot[2])
+    r = dict(ray_seg_offset=seg_off, ray_sample_offset=samp_off, max_samples_per_ray=maxs,
              seg_node=torch.empty(K, dtype=_I32, device=dev), seg_ray=torch.empty(K, dtype=_I32, device=dev),
              seg_count=torch.empty(K, dtype=_I32, device=dev), seg_tmin=torch.empty(K, dtype=_F32, device=dev),
              seg_tmax=torch.empty(K, dtype=_F32, device=dev), seg_sample_offset=torch.empty(K, dtype=_I64, device=dev))
-    call("neat_raysample_segments", s, ptr(pos_min), ptr(pos_max), ptr(flags), num_nodes, max_depth, ptr(o), ptr(d), R,
-         max_spn, ptr(seg_off), ptr(samp_off), K, ptr(r["seg_node"]), ptr(r["seg_ray"]), ptr(r["seg_count"]),
-         ptr(r["seg_tmin"]), ptr(r["seg_tmax"]), ptr(r["seg_sample_offset"]))
+    cols = (ptr(r["seg_node"]), ptr(r["seg_ray"]), ptr(r["seg_count"]), ptr(r["seg_tmin"]), ptr(r["seg_tmax"]),
+            ptr(r["seg_sample_offset"]))
+    if single_walk:
+        call("neat_raysample_compact", s, ptr(pos_min), ptr(pos_max), ptr(flags), num_nodes, max_depth, ptr(o), ptr(d),
+             R, max_spn, ptr(seg_off), ptr(samp_off), ptr(scratch), sc_bytes, K, *cols)
+    else:
+        call("neat_raysample_segments", s, ptr(pos_min), ptr(pos_max), ptr(flags), num_nodes, max_depth, ptr(o),
+             ptr(d), R, max_spn, ptr(seg_off), ptr(samp_off), K, *cols)
     if emit:
         r.update(t=torch.empty(S, dtype=_F32, device=dev), weight=torch.empty(S, dtype=_F32, device=dev),
                  coord=torch.empty(S, 3, dtype=_F32, device=dev), ray_index=torch.empty(S, dtype=_I64, device=dev),
@@ -159,11 +171,15 @@ def mse_fp64(pred, target, grad_scale=1.0, want_grad=True, stream=None):
     return loss, grad
 
 
-def grid_to_bf16(grid, stream=None):
-    """Round-to-nearest-even BF16 bit patterns (int16) of an FP32 grid, same shape and strides (neat_grid_to_bf16)."""
+def grid_to_bf16(grid, stream=None, out=None):
+    """Round-to-nearest-even BF16 bit patterns (int16) of an FP32 grid, same shape and strides (neat_grid_to_bf16);
+    written into `out` when given (the per-optimizer-step refresh of an existing copy)."""
     if not grid.is_contiguous(memory_format=torch.channels_last_3d) and not grid.is_contiguous():
         raise ValueError("grid_to_bf16: grid must be dense")
-    out = torch.empty_like(grid, dtype=torch.int16)
+    if out is None:
+        out = torch.empty_like(grid, dtype=torch.int16)
+    elif out.dtype != torch.int16 or out.shape != grid.shape or out.stride() != grid.stride():
+        raise ValueError("grid_to_bf16: out must be int16 with the grid's shape and strides")
     call("neat_grid_to_bf16", stream_ptr(stream), ptr(grid), grid.numel(), ptr(out))
     return out
 

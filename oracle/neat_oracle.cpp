@@ -690,11 +690,12 @@ void projector_bwd(const SampleCtx<T>& c, const SegList& s, const GridDesc& g, c
     }
     if (nt > 1)
     {
+        // every entry sums the threads' partials in thread order (parallel over entries; the order per entry is fixed)
+#pragma omp parallel for schedule(static) num_threads(nt)
+        for (int64_t i = 0; i < grid_numel; ++i)
+            for (int t = 0; t < nt; ++t) grad_grid[i] += priv_grid[t][i];
         for (int t = 0; t < nt; ++t)
-        {
-            for (int64_t i = 0; i < grid_numel; ++i) grad_grid[i] += priv_grid[t][i];
             for (int64_t i = 0; i < P; ++i) grad_params[i] += priv_par[t][i];
-        }
     }
 }
 

@@ -5,7 +5,9 @@ segment list, the FP32 and FP64 projections with jitter, and the gradients of su
 
 The reference ships no test vectors, so these are the oracle's own outputs, frozen: a later change to the oracle or
 to the HIP path shows up as a difference against them. Regenerate (only on purpose) from the repository root with
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [small] [c2]
+
+projector_c2.npz holds the same kind of results for 4096 rays of the full-size configuration (make_c2).
 """
 import os
 import sys
@@ -83,8 +85,51 @@ def make():
     return out
 
 
+C2_RAYS = 4096
+C2_SEED = 31
+C2_GRID_ENTRIES = 4096
+
+
+def make_c2():
+    """The full-size configuration (W.C2: depth-5 tree adapted to the phantom, 4096 leaves, 17^3 x 8 grids, decoder
+    8 -> 64 -> 64 -> 1) rebuilt from its seeds, on 4096 of its rays with jitter: the canonical segment list, and from
+    the FP32 and the FP64 oracle the projections, the decoder gradient and, of the 644 MB grid gradient, the per-slot
+    L2 norms and 4096 sampled entries. The scene itself is not stored: W.make_scene(W.C2) regenerates it."""
+    cfg = W.C2
+    sc = W.make_scene(cfg)
+    o, d = W.batch_rays(cfg, C2_RAYS, seed=C2_SEED)
+    segs = O.raysample(sc["tree"], sc["active_ids"], o, d, cfg.max_spn, emit=False)
+    rng = np.random.default_rng(C2_SEED)
+    gproj = rng.normal(size=o.shape[0]).astype(np.float32)
+    pick = rng.choice(sc["grid"].size, C2_GRID_ENTRIES, replace=False).astype(np.int64)
+    mlp = dict(sc["mlp"], params=sc["params"])
+    out = dict(origin=o, direction=d, grad_proj=gproj, grid_pick=pick, jitter_seed=np.int64(C2_SEED))
+    for k in ("ray_seg_offset", "ray_sample_offset", "seg_node", "seg_ray", "seg_count", "seg_tmin", "seg_tmax"):
+        out[k] = segs[k]
+    for prec, tag in ((0, "fp32"), (1, "fp64")):
+        out[f"proj_{tag}"] = O.projector_fwd(sc["tree"], sc["prefix"], o, d, segs, sc["grid"], mlp, jitter=True,
+                                             seed=C2_SEED, prec=prec).astype(np.float32 if prec == 0 else np.float64)
+    for prec, tag in ((0, "fp32"), (1, "fp64")):
+        gg, gp = O.projector_bwd(sc["tree"], sc["prefix"], o, d, segs, sc["grid"], mlp, gproj.astype(np.float64),
+                                 jitter=True, seed=C2_SEED, prec=prec, nthreads=min(8, O.max_threads()))
+        dt = np.float32 if prec == 0 else np.float64
+        gg = gg.astype(dt)
+        out[f"grad_params_{tag}"] = gp.astype(dt)
+        out[f"grad_grid_slot_norm_{tag}"] = np.sqrt((gg.reshape(gg.shape[0], -1).astype(np.float64) ** 2).sum(1))
+        out[f"grad_grid_pick_{tag}"] = gg.reshape(-1)[pick]
+        del gg
+    return out
+
+
 if __name__ == "__main__":
-    out = make()
-    path = os.path.join(HERE, "projector_small.npz")
-    np.savez_compressed(path, **out)
-    print(path, {k: getattr(v, "shape", ()) for k, v in out.items()})
+    which = sys.argv[1:] or ["small", "c2"]
+    if "small" in which:
+        out = make()
+        path = os.path.join(HERE, "projector_small.npz")
+        np.savez_compressed(path, **out)
+        print(path, {k: getattr(v, "shape", ()) for k, v in out.items()})
+    if "c2" in which:
+        out = make_c2()
+        path = os.path.join(HERE, "projector_c2.npz")
+        np.savez_compressed(path, **out)
+        print(path, {k: getattr(v, "shape", ()) for k, v in out.items()})

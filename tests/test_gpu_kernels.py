@@ -41,13 +41,13 @@ def special_rays():
     return o, d
 
 
-def run_sampler(scene, o, d, max_spn, jitter=False, seed=0):
+def run_sampler(scene, o, d, max_spn, jitter=False, seed=0, single_walk=True):
     t = scene["tree"]
     pmin = torch.from_numpy(t["pos_min"]).to(DEV)
     pmax = torch.from_numpy(t["pos_max"]).to(DEV)
     flags = ops.tree_flags(scene["active"], t["max_depth"]).to(DEV)
     r = ops.raysample(pmin, pmax, flags, t["max_depth"], torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV),
-                      max_spn, jitter=jitter, seed=seed)
+                      max_spn, jitter=jitter, seed=seed, single_walk=single_walk)
     torch.cuda.synchronize()
     return {k: (v.cpu().numpy() if torch.is_tensor(v) else v) for k, v in r.items()}
 
@@ -79,14 +79,48 @@ def test_sampler_bitexact_uniform_tree(oracle, jitter):
     compare_sampler(gpu, ref)
 
 
-def test_sampler_bitexact_adaptive_tree(oracle):
-    print("case: sampler C2 geometry (4096 adaptive leaves), 4096 rays", flush=True)
+@pytest.mark.parametrize("single_walk", [True, False])
+def test_sampler_bitexact_adaptive_tree(oracle, single_walk):
+    print("case: sampler C2 geometry (4096 adaptive leaves), 4096 rays, single walk", single_walk, flush=True)
     sc = W.make_scene(W.C2)
     o, d = W.batch_rays(W.C2, 4096, seed=2)
-    gpu = run_sampler(sc, o, d, 64)
+    gpu = run_sampler(sc, o, d, 64, single_walk=single_walk)
     ref = oracle.raysample(sc["tree"], sc["active_ids"], o, d, 64)
     compare_sampler(gpu, ref)
     assert gpu["seg_node"].size > 10000
+
+
+def test_sampler_bitexact_full_batch_scale(oracle):
+    """A quarter of a full 2^18-ray batch of the full-size configuration (65,536 rays, about 0.49 M segments and
+    13 M samples): every segment, count, t and weight bit for bit against the oracle's brute force over all active
+    leaves, with jitter."""
+    print("case: sampler C2 geometry, 65536 rays, jitter", flush=True)
+    sc = W.make_scene(W.C2)
+    o, d = W.batch_rays(W.C2, 1 << 16, seed=12)
+    gpu = run_sampler(sc, o, d, 64, jitter=True, seed=9)
+    ref = oracle.raysample(sc["tree"], sc["active_ids"], o, d, 64, jitter=True, seed=9)
+    compare_sampler(gpu, ref)
+    assert gpu["seg_node"].size > 400000
+
+
+def test_sampler_single_walk_rewalks_rays_beyond_the_scratch(oracle):
+    """Every node of a depth-3 tree active (nested): rays cross more segments than the single walk's per-ray scratch
+    holds (3 * 2^3 + 8 = 32), and those rays are walked again by the compaction; the list stays bit-exact."""
+    print("case: sampler, all nodes of a depth-3 tree active", flush=True)
+    t = W.build_tree(3)
+    act = np.ones(t["depth"].shape[0], np.int32)
+    ids, prefix = W.update_active(act)
+    sc = dict(tree=t, active=act, active_ids=ids)
+    o, d = W.batch_rays(W.C1, 4096, seed=7)
+    # near-diagonal rays cross up to 22 + 10 + 4 + 1 nested boxes
+    rng = np.random.default_rng(3)
+    do = (np.array([-2.0, -2.0, -2.0]) + rng.uniform(-0.05, 0.05, (512, 3))).astype(np.float32)
+    dd = np.array([1.0, 1.0, 1.0]) + rng.uniform(-0.08, 0.08, (512, 3))
+    dd = (dd / np.linalg.norm(dd, axis=1, keepdims=True)).astype(np.float32)
+    o, d = np.concatenate([o, do]), np.concatenate([d, dd])
+    gpu = run_sampler(sc, o, d, 8)
+    assert np.diff(gpu["ray_seg_offset"]).max() > 32
+    compare_sampler(gpu, oracle.raysample(t, ids, o, d, 8))
 
 
 def test_sampler_nested_active_and_empty(oracle):
