@@ -85,7 +85,7 @@ class Runner:
     (geometry_ex_ex.cpp:40-49, geometry.h:66-88)."""
 
     def __init__(self, a, cfg, ds, batches, path, precision, prop=None, grid_bf16=None, train=False,
-                 active_ids=None):
+                 active_ids=None, ray_grad=False):
         import torch
         self.a, self.cfg, self.ds, self.batches = a, cfg, ds, batches
         self.path, self.precision, self.prop, self.grid_bf16, self.train = path, precision, prop, grid_bf16, train
@@ -96,9 +96,12 @@ class Runner:
             names = ["sample", "proposal", "proposal_train"] + names[1:]
         if a.regularizer and path == "fused":
             names = names + ["regularizer"]
+        if ray_grad:
+            names = names + ["ray_grad"]
         if train:
             names = names + ["optimizer"]
         self.stage_names = names
+        self.ray_grad = ray_grad
         self.events = {k: [] for k in names}
         self.gg_buf = torch.empty_like(ds.grid)
         self.gp_buf = torch.empty_like(ds.params)
@@ -143,6 +146,10 @@ class Runner:
                 rec(name, 0)
                 pj.backward_stage(st, gproj, self.gg_buf, self.gp_buf)
                 rec(name, 1)
+            if self.ray_grad:
+                rec("ray_grad", 0)
+                pj.ray_grad()
+                rec("ray_grad", 1)
             if a.regularizer:
                 rec("regularizer", 0)
                 ops.volume_regularizer(ds.pos_min, ds.pos_max, ds.flags, ds.prefix, self.active_ids, ds.grid, seed=i)
@@ -313,6 +320,13 @@ def main():
                                "samples_per_batch": round(ns), "ratio": round(rays_per_s / (cfg.batch / (ms / 1e3)), 2)}
             del r
             torch.cuda.empty_cache()
+        # pose refinement: the headline form with the positional gradient onto the rays (fused, as the rays of
+        # GetRays with pose gradients take it)
+        r = Runner(a, cfg, ds, batches, "fused", "bf16", prop, grid_bf16, ray_grad=True)
+        ms, ns, sm = r.run(a.compare_steps, 1)
+        baselines["bf16_proposal_with_ray_grad"] = {"rays_per_s": round(cfg.batch / (ms / 1e3), 1),
+                                                    "ms_per_step": round(ms, 3), "ray_grad_ms": round(sm["ray_grad"], 3)}
+        del r
         vs_baseline = baselines["fp32_v1_unfused"]["ratio"]
         baselines["denominator"] = "fp32_v1_unfused (the first plain-FP32 GPU version: the unfused kernel chain)"
         # the training step: the headline form followed by Adam on the grid and the decoder and the BF16 refresh,

@@ -292,6 +292,16 @@ int neat_projector_bwd_stage(neat_stream_t stream, const neat_projector_args* ar
                              const float* grad_proj, float* grad_grid, float* grad_params, void* workspace,
                              size_t workspace_bytes);
 
+/* Gradients of sum_r grad_proj[r] * proj[r] with respect to the rays' origins and directions, grad_origin and
+ * grad_direction [num_rays, 3] (either may be NULL), the samples' t and weights held fixed: the positional gradient
+ * the reference's chain yields for rays that require grad (the o + d t recompute, HyperTree.cu:426-432; the local
+ * coordinates' backward, HyperTree.cu:116-199; the grid sample's uv gradient, IndirectGridSample3D.cu:752-790; the
+ * index_select backward onto the rays). Reads the dL/dfeature of the decoder backward: call it after
+ * neat_projector_bwd (or its stage 1) on the same workspace and arguments. Deterministic: per-segment sums in sample
+ * order, then per-ray sums in canonical segment order. */
+int neat_projector_ray_grad(neat_stream_t stream, const neat_projector_args* args, float* grad_origin,
+                            float* grad_direction, void* workspace, size_t workspace_bytes);
+
 /* Per-sample decoded values softplus(decoder(trilinear(grid))) (unweighted) of the planned segment list, in the
  * plan's node-major sample order (values [num_samples]), and seg_value_offset [num_segments] = the first sample of each
  * canonical segment in that order. Matrix-core kernels only (8 channels, NEAT_PREC_FP32 or NEAT_PREC_BF16). */

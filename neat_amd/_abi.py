@@ -82,6 +82,7 @@ SIGNATURES.update({
     "neat_projector_plan": (ctypes.c_int, [_P, _PA, _P, _SZ]),
     "neat_projector_fwd": (ctypes.c_int, [_P, _PA, _I32, _P, _P, _SZ]),
     "neat_projector_bwd": (ctypes.c_int, [_P, _PA, _P, _P, _P, _P, _SZ]),
+    "neat_projector_ray_grad": (ctypes.c_int, [_P, _PA, _P, _P, _P, _SZ]),
     "neat_grid_to_bf16": (ctypes.c_int, [_P, _P, _I64, _P]),
     "neat_projector_bwd_stage": (ctypes.c_int, [_P, _PA, _I32, _P, _P, _P, _P, _SZ]),
     "neat_projector_sample_values": (ctypes.c_int, [_P, _PA, _P, _P, _P, _SZ]),

@@ -39,8 +39,10 @@ struct SegmentList
     torch::Tensor tmin;               // float [num_segments]
     torch::Tensor tmax;               // float [num_segments]
     torch::Tensor sample_offset;      // long [num_segments]
-    torch::Tensor ray_origin;         // the rays the segments belong to
+    torch::Tensor ray_origin;         // the rays the segments belong to (FP32, contiguous, detached)
     torch::Tensor ray_direction;
+    torch::Tensor ray_origin_in;      // the caller's ray tensors themselves: gradients flow to these
+    torch::Tensor ray_direction_in;
     int64_t num_samples = 0;
     bool jitter         = false;
     uint64_t seed       = 0;

@@ -203,8 +203,10 @@ SampleList HyperTreeBaseImpl::create_samples(const RayList& rays, int max_spn, b
     seg.num_samples   = S;
     seg.jitter        = jitter;
     seg.seed          = jitter ? (uint64_t)torch::randint(0, (int64_t)1 << 62, {1}, torch::kLong).item<int64_t>() : 0;
-    seg.ray_origin    = o;
-    seg.ray_direction = d;
+    seg.ray_origin       = o;
+    seg.ray_direction    = d;
+    seg.ray_origin_in    = rays.origin;
+    seg.ray_direction_in = rays.direction;
 
     SampleList list;
     list.max_samples_per_ray = (int)totals[2].item<int64_t>();

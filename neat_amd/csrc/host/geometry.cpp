@@ -92,9 +92,15 @@ struct ProjectorFn : public torch::autograd::Function<ProjectorFn>
         return a;
     }
 
-    static torch::Tensor forward(AutogradContext* ctx, torch::Tensor grid, torch::Tensor params,
-                                 std::vector<torch::Tensor> geo, std::vector<int64_t> iv, double beta, bool compensated)
+    // origin_in / direction_in: the caller's rays (gradients flow to them when they require grad); geo holds their
+    // FP32 contiguous copies the kernels read
+    static torch::Tensor forward(AutogradContext* ctx, torch::Tensor grid, torch::Tensor params, torch::Tensor origin_in,
+                                 torch::Tensor direction_in, std::vector<torch::Tensor> geo, std::vector<int64_t> iv,
+                                 double beta, bool compensated)
     {
+        ctx->saved_data["ray_grad"] = origin_in.requires_grad() || direction_in.requires_grad();
+        ctx->saved_data["o_type"]   = (int64_t)origin_in.scalar_type();
+        ctx->saved_data["d_type"]   = (int64_t)direction_in.scalar_type();
         std::vector<torch::Tensor> t = {grid, params.contiguous()};
         for (auto& g : geo) t.push_back(g);
         auto s = current_stream();
@@ -136,9 +142,19 @@ struct ProjectorFn : public torch::autograd::Function<ProjectorFn>
         NEAT_CALL(neat_projector_bwd(current_stream(), &a, dptr<float>(gproj), dptr<float>(gg), dptr<float>(gp),
                                      ws.data_ptr(), (size_t)ws.numel()));
         // one slot per forward argument (each tensor of `geo` counts); trailing slots stay undefined
-        tensor_list out(2 + 11 + 3);
+        tensor_list out(4 + 11 + 3);
         out[0] = gg;
         out[1] = gp;
+        if (ctx->saved_data["ray_grad"].toBool())
+        {
+            // the positional gradient onto the rays, from the same backward's dL/dfeature (HyperTree.cu:426-432)
+            auto go = torch::empty({a.num_rays, 3}, t[0].options().memory_format(at::MemoryFormat::Contiguous));
+            auto gd = torch::empty_like(go);
+            NEAT_CALL(neat_projector_ray_grad(current_stream(), &a, dptr<float>(go), dptr<float>(gd), ws.data_ptr(),
+                                              (size_t)ws.numel()));
+            out[2] = go.to((c10::ScalarType)ctx->saved_data["o_type"].toInt());
+            out[3] = gd.to((c10::ScalarType)ctx->saved_data["d_type"].toInt());
+        }
         return out;
     }
 };
@@ -181,11 +197,9 @@ torch::Tensor HierarchicalNeuralGeometry::DecodeFeatures(torch::Tensor neural_fe
 
 torch::Tensor HierarchicalNeuralGeometry::ComputeImage(SampleList all_samples, int num_channels, int num_pixels)
 {
-    // the fused projector returns no positional gradient: samples whose coordinates carry one (rays that require
-    // grad, e.g. from GetRays with pose refinement) take the kernel chain, whose grid sample backward yields d uv
-    const bool positional_grad = all_samples.global_coordinate.defined() && all_samples.global_coordinate.requires_grad();
-    if (params->train_params.use_fused_projector && all_samples.segments.defined() && num_channels == 1 &&
-        !positional_grad)
+    // rays that require grad (e.g. GetRays with pose refinement) keep the fused projector: its backward also returns
+    // the positional gradient onto the rays (neat_projector_ray_grad)
+    if (params->train_params.use_fused_projector && all_samples.segments.defined() && num_channels == 1)
     {
         auto img = ComputeImageFused(all_samples, num_pixels);
         if (img.defined()) return img;
@@ -464,7 +478,10 @@ torch::Tensor GeometryExEx::ComputeImageFused(const SampleList& samples, int num
     std::vector<int64_t> iv = {s.num_samples, s.jitter ? 1 : 0, (int64_t)s.seed, np.grid_features,
                                np.decoder_hidden_features, np.decoder_hidden_layers, 1, act_id(np.decoder_activation),
                                NEAT_ACT_SOFTPLUS, precision_id(params->train_params.projector_precision)};
-    return ProjectorFn::apply(explicit_grid_generator->grid_data, decoder->packed_parameters(), geo, iv,
+    // the caller's rays carry the positional gradient; without them (a SampleList assembled by hand) the FP32 copies
+    auto o_in = s.ray_origin_in.defined() ? s.ray_origin_in : s.ray_origin;
+    auto d_in = s.ray_direction_in.defined() ? s.ray_direction_in : s.ray_direction;
+    return ProjectorFn::apply(explicit_grid_generator->grid_data, decoder->packed_parameters(), o_in, d_in, geo, iv,
                               (double)np.softplus_beta, params->train_params.compensated_sum);
 }
 

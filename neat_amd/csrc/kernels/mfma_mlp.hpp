@@ -316,6 +316,8 @@ struct WeightLDSRelu : WeightLDSReluFwd<H>
     static constexpr int MT = H / 32;
     bf16x8 w2t[MT][MT][2][64];
     bf16x8 w1t[MT][2][64];
+    // output layer as an A fragment: row 0 holds w3 in the K order of an h fragment of tile mt, step st; other rows 0
+    bf16x8 w3f[MT][2][64];
 };
 
 // (hi, lo) BF16 split of an FP32 value into slots k, k + 1 of a fragment: hi + lo carries about 16 significant bits
@@ -376,6 +378,16 @@ __device__ void load_weights_relu(const float* __restrict__ params, WS& w)
 #pragma unroll
         for (int j = 0; j < 8; ++j) f[j] = (__bf16)(c < C ? W0[(32 * ms + crow(st * 8 + j, g)) * C + c] : 0.f);
         w.w1t[ms][st][l] = f;
+    }
+    if constexpr (kBwd)
+    for (int t = threadIdx.x; t < MT * 2 * 64; t += blockDim.x)
+    {
+        const int l = t % 64, st = (t / 64) % 2, mt = t / 128;
+        const int g = l >> 5, m = l & 31;
+        bf16x8 f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) f[j] = (__bf16)(m == 0 ? W2[32 * mt + crow(st * 8 + j, g)] : 0.f);
+        w.w3f[mt][st][l] = f;
     }
     for (int t = threadIdx.x; t < 64; t += blockDim.x) w.w3[t] = t < H ? W2[t] : 0.f;
     if (threadIdx.x == 0) w.b3 = b2[0];

@@ -445,34 +445,26 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
                     }
                 }
             };
-            constexpr int kSB = 4;
-            float4 ga[kSB], da[kSB];
-            auto load = [&](int64_t s0) {
-#pragma unroll
-                for (int b = 0; b < kSB; ++b)
-                {
-                    if (s0 + b < e2)
-                    {
-                        ga[b] = sgc[base + s0 + b];
-                        da[b] = dh[s0 + b];
-                    }
-                }
-            };
-            load(s2);
-            for (int64_t s0 = s2; s0 < e2; s0 += kSB)
+            // blocks of 8 samples: 128 B of each array per stream and block, a whole cache line of each once the
+            // stream is aligned to it (its first block runs to the next 8-sample boundary), so a line is read once
+            constexpr int kSB = 8;
+            for (int64_t s0 = s2; s0 < e2;)
             {
+                const int64_t s1 = min(e2, ((base + s0) / kSB + 1) * kSB - base);
                 float4 gb[kSB], db[kSB];
 #pragma unroll
                 for (int b = 0; b < kSB; ++b)
                 {
-                    gb[b] = ga[b];
-                    db[b] = da[b];
+                    if (s0 + b < s1)
+                    {
+                        gb[b] = sgc[base + s0 + b];
+                        db[b] = dh[s0 + b];
+                    }
                 }
-                load(s0 + kSB);  // the next block in flight while this one is summed
 #pragma unroll
                 for (int b = 0; b < kSB; ++b)
                 {
-                    if (s0 + b >= e2) break;
+                    if (s0 + b >= s1) break;
                     const float4 gc = gb[b];
                     // the trilinear corners of corners_ix, the same operations
                     const int x0 = (int)floorf(gc.x), y0 = (int)floorf(gc.y), z0 = (int)floorf(gc.z);
@@ -504,6 +496,7 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
                         acc[k][3] += wk * dv.w;
                     }
                 }
+                s0 = s1;
             }
             if (cell >= 0) flush();
         }
@@ -969,17 +962,18 @@ Layout proj_layout(const ProjSizes& z)
     L.add(4 * (kNodeWGMax + 1));          // 16 first chunk of each node-resident workgroup
     L.add(64 * (size_t)z.K);              // 17 node-major segment records (ot, dt, i, h)
     L.add(z.prop ? 32 * (size_t)z.K : 16);  // 18 node-major proposal cells / CDF rows
+    L.add(24 * (size_t)z.K);              // 19 per-segment (dL/do, dL/dd) of the positional gradient
     return L;
 }
 
-// Node-resident workgroups: one per CU of the current device (at most kNodeWGMax).
+// Node-resident workgroups: kNodeWGPerCU per CU of the current device (at most kNodeWGMax).
 int node_workgroups()
 {
     int dev = 0, cu = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) !=
                                                 hipSuccess || cu <= 0)
-        return 256;
-    return std::min(cu, kNodeWGMax);
+        cu = 256;
+    return std::min(cu * kNodeWGPerCU, kNodeWGMax);
 }
 
 int bw1_workgroups(const ProjSizes& z) { return (int)std::max<int64_t>(1, std::min<int64_t>(z.max1, 2048)); }
@@ -1271,6 +1265,163 @@ int dispatch_bwd(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b, 
     return H == 64 ? launch_bw_valu<4, 64>(s, a, p, b, stage) : launch_bw_valu<4, 32>(s, a, p, b, stage);
 }
 
+// ---- positional gradient (rays that require grad) -------------------------------------------------------------------
+// Sample j of the segment at node-major position q, from the plan's record: t and the local coordinate with the
+// operations of ctx_sample, so the same bits.
+__device__ __forceinline__ void record_sample(const ProjGeom& g, const Plan& p, int64_t q, int32_t j,
+                                              const NodeBox& nb, float& tt, float (&uv)[3])
+{
+    const float4 ot = p.rec_ot[q], dt = p.rec_dt[q];
+    const int4 ri   = p.rec_i[q];
+    const uint4 rh  = p.rec_h[q];
+    const float tmin = ot.w, tmax = dt.w, step = __uint_as_float(rh.w);
+    const int32_t rank = ri.z, n = ri.w;
+    const uint64_t hr  = ((uint64_t)rh.y << 32) | rh.x;
+    auto a_of          = [&](int32_t jj) { return g.jitter ? jitter_a_h(hr, rank, jj) : 0.5f; };
+    if (rh.z != 0)
+    {
+        float cell[8];
+        const float4 u = p.rec_c[2 * q], v = p.rec_c[2 * q + 1];
+        cell[0] = u.x; cell[1] = u.y; cell[2] = u.z; cell[3] = u.w;
+        cell[4] = v.x; cell[5] = v.y; cell[6] = v.z; cell[7] = v.w;
+        float t1, t2;
+        if (rh.z == 1)
+        {
+            t1 = j == 0 ? tmin : cell[j - 1];
+            t2 = j + 1 >= n ? tmax : cell[j];
+        }
+        else
+        {
+            t1 = proposal_cell(cell, tmin, tmax, n, j);
+            t2 = proposal_cell(cell, tmin, tmax, n, j + 1);
+        }
+        const float a = a_of(j);
+        tt            = t1 * (1.0f - a) + t2 * a;
+    }
+    else
+    {
+        tt = sample_t(tmin, step, j, a_of(j));
+    }
+    const float o[3] = {ot.x, ot.y, ot.z}, d[3] = {dt.x, dt.y, dt.z};
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+    {
+        const float pd  = o[e] + d[e] * tt;
+        const float rel = nb.rinv[e] != 0.f ? (pd - nb.mn[e]) * nb.rinv[e] : (pd - nb.mn[e]) / (nb.mx[e] - nb.mn[e]);
+        uv[e]           = rel * 2 - 1;
+    }
+}
+
+// One thread per node-major segment: for every sample, the uv gradient of the trilinear gather contracted with the
+// backward's dL/dfeature (IndirectGridSample3D.cu:752-790, the terms in the reference's order), through the local
+// coordinates' backward (grad * 2) / (max - min) onto the position o + d t, summed over the segment's samples in j
+// order into (dL/do, dL/dd); written at the canonical segment index. Segments without a grid slot get zeros.
+// kHalfMajor: dL/dfeature laid out [C/4][S][4] (the matrix-core backward), else [S][C].
+template <int C, bool kBF16, bool kHalfMajor>
+__global__ void __launch_bounds__(256) k_proj_ray_grad(ProjGeom g, Plan p, const void* grid_lowp, const float* dfeat,
+                                                       float* seg_grad)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.K) return;
+    const int4 ri     = p.rec_i[q];
+    const int32_t k   = ri.x, n = ri.w;
+    const int32_t node = g.seg_node[k];
+    const int64_t slot = (node >= 0 && node < g.num_nodes) ? g.prefix[node] : -1;
+    float go[3] = {0.f, 0.f, 0.f}, gd[3] = {0.f, 0.f, 0.f};
+    if (slot >= 0 && slot < g.N)
+    {
+        NodeBox nb;
+        node_bounds(g, node, nb);
+        const int32_t G     = g.G;
+        const int64_t nvox  = (int64_t)G * G * G * C;
+        const float* gf     = g.grid + slot * nvox;
+        const unsigned short* gh = static_cast<const unsigned short*>(grid_lowp) + slot * nvox;
+        const int64_t s0    = p.nm_soff[q];
+        for (int32_t j = 0; j < n; ++j)
+        {
+            float tt, uv[3];
+            record_sample(g, p, q, j, nb, tt, uv);
+            const float ix = grid_coord(uv[0], G), iy = grid_coord(uv[1], G), iz = grid_coord(uv[2], G);
+            const Corners cr = corners_ix(ix, iy, iz, G, C);
+            const float x0 = floorf(ix), y0 = floorf(iy), z0 = floorf(iz);
+            const float x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+            float gix = 0.f, giy = 0.f, giz = 0.f;
+            const int64_t smp = s0 + j;
+#pragma unroll
+            for (int ch = 0; ch < C; ++ch)
+            {
+                const float gout = kHalfMajor ? dfeat[((int64_t)(ch >> 2) * g.S + smp) * 4 + (ch & 3)]
+                                              : dfeat[smp * C + ch];
+                float val[8];
+#pragma unroll
+                for (int c8 = 0; c8 < 8; ++c8)
+                    val[c8] = kBF16 ? bf16_to_f32(gh[cr.off[c8] + ch]) : gf[cr.off[c8] + ch];
+                const float m1 = -1.f;
+                gix += m1 * val[0] * (y1 - iy) * (z1 - iz) * gout;
+                gix += val[1] * (y1 - iy) * (z1 - iz) * gout;
+                gix += m1 * val[2] * (iy - y0) * (z1 - iz) * gout;
+                gix += val[3] * (iy - y0) * (z1 - iz) * gout;
+                gix += m1 * val[4] * (y1 - iy) * (iz - z0) * gout;
+                gix += val[5] * (y1 - iy) * (iz - z0) * gout;
+                gix += m1 * val[6] * (iy - y0) * (iz - z0) * gout;
+                gix += val[7] * (iy - y0) * (iz - z0) * gout;
+
+                giy += m1 * val[0] * (x1 - ix) * (z1 - iz) * gout;
+                giy += m1 * val[1] * (ix - x0) * (z1 - iz) * gout;
+                giy += val[2] * (x1 - ix) * (z1 - iz) * gout;
+                giy += val[3] * (ix - x0) * (z1 - iz) * gout;
+                giy += m1 * val[4] * (x1 - ix) * (iz - z0) * gout;
+                giy += m1 * val[5] * (ix - x0) * (iz - z0) * gout;
+                giy += val[6] * (x1 - ix) * (iz - z0) * gout;
+                giy += val[7] * (ix - x0) * (iz - z0) * gout;
+
+                giz += m1 * val[0] * (x1 - ix) * (y1 - iy) * gout;
+                giz += m1 * val[1] * (ix - x0) * (y1 - iy) * gout;
+                giz += m1 * val[2] * (x1 - ix) * (iy - y0) * gout;
+                giz += m1 * val[3] * (ix - x0) * (iy - y0) * gout;
+                giz += val[4] * (x1 - ix) * (y1 - iy) * gout;
+                giz += val[5] * (ix - x0) * (y1 - iy) * gout;
+                giz += val[6] * (x1 - ix) * (iy - y0) * gout;
+                giz += val[7] * (ix - x0) * (iy - y0) * gout;
+            }
+            const float guv[3] = {gix * (float)(G - 1) / 2, giy * (float)(G - 1) / 2, giz * (float)(G - 1) / 2};
+#pragma unroll
+            for (int e = 0; e < 3; ++e)
+            {
+                const float gpos = (guv[e] * 2) / (nb.mx[e] - nb.mn[e]);
+                go[e] += gpos;
+                gd[e] += gpos * tt;
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+    {
+        seg_grad[6 * (int64_t)k + e]     = go[e];
+        seg_grad[6 * (int64_t)k + 3 + e] = gd[e];
+    }
+}
+
+// dL/do, dL/dd of every ray: its segments' sums in canonical order.
+__global__ void __launch_bounds__(256) k_ray_grad_sum(const int64_t* ray_seg_offset, int64_t R, const float* seg_grad,
+                                                      float* grad_origin, float* grad_direction)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int64_t k = ray_seg_offset[r]; k < ray_seg_offset[r + 1]; ++k)
+    {
+#pragma unroll
+        for (int e = 0; e < 6; ++e) acc[e] += seg_grad[6 * k + e];
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+    {
+        if (grad_origin) grad_origin[3 * r + e] = acc[e];
+        if (grad_direction) grad_direction[3 * r + e] = acc[3 + e];
+    }
+}
+
 // first node-major sample of every canonical segment
 __global__ void __launch_bounds__(256) k_seg_value_offset(const int32_t* perm, const int64_t* nm_soff, int64_t K,
                                                           int64_t* off)
@@ -1451,6 +1602,46 @@ extern "C"
         b.grad_grid     = grad_grid;
         b.grad_params   = grad_params;
         return dispatch_bwd(s, a, p, b, stage);
+    }
+
+    int neat_projector_ray_grad(neat_stream_t stream, const neat_projector_args* x, float* grad_origin,
+                                float* grad_direction, void* ws, size_t bytes)
+    {
+        Args a;
+        int st = make_args(a, x);
+        if (st) return st;
+        if ((st = check_ws(a, ws, bytes))) return st;
+        const int64_t R = x->num_rays;
+        NEAT_REQUIRE(R == 0 || grad_origin || grad_direction, "neat_projector_ray_grad: no output");
+        if (R == 0) return NEAT_OK;
+        hipStream_t s = as_stream(stream);
+        const auto& z = a.z;
+        float* seg_grad = reinterpret_cast<float*>(region(ws, z, 19));
+        if (z.K > 0)
+        {
+            Plan p             = plan_view(ws, z);
+            const float* dfeat = reinterpret_cast<const float*>(region(ws, z, 13));
+            const bool bf16    = a.precision == NEAT_PREC_BF16;
+            const unsigned nb  = grid_for(z.K, 256);
+            if (z.C == 8)
+            {
+                if (a.mfma && bf16)
+                    k_proj_ray_grad<8, true, true><<<nb, 256, 0, s>>>(a.g, p, a.grid_bf16, dfeat, seg_grad);
+                else if (a.mfma)
+                    k_proj_ray_grad<8, false, true><<<nb, 256, 0, s>>>(a.g, p, nullptr, dfeat, seg_grad);
+                else
+                    k_proj_ray_grad<8, false, false><<<nb, 256, 0, s>>>(a.g, p, nullptr, dfeat, seg_grad);
+            }
+            else
+            {
+                k_proj_ray_grad<4, false, false><<<nb, 256, 0, s>>>(a.g, p, nullptr, dfeat, seg_grad);
+            }
+            NEAT_LAUNCHED();
+        }
+        k_ray_grad_sum<<<grid_for(R, 256), 256, 0, s>>>(x->ray_seg_offset, R, seg_grad, grad_origin,
+                                                         grad_direction);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
     }
 
     int neat_projector_bwd(neat_stream_t stream, const neat_projector_args* x, const float* grad_proj,

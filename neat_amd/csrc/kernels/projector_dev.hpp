@@ -15,12 +15,14 @@ constexpr int kSegCost    = 16;    // chunk cost of a segment, in samples
 constexpr int kSegCap     = kFwdChunk / kSegCost;  // segments per forward chunk, at most
 constexpr int kFwdThreads = 256;
 
-// Node-resident workgroups (the BF16 relu kernels): one workgroup of kNodeThreads per CU walks a contiguous,
+// Node-resident workgroups (the BF16 relu kernels): a workgroup of kNodeThreads (one fits a CU) walks a contiguous,
 // cost-balanced run of forward chunks and keeps the current slot's BF16 node grid in LDS, so every corner gather
-// reads LDS and each node grid is fetched from HBM about once per pass.
-constexpr int kNodeThreads = 512;
-constexpr int kNodeWaves   = kNodeThreads / 64;
-constexpr int kNodeWGMax   = 512;   // workgroups a plan can describe
+// reads LDS and each node grid is fetched from HBM about once per pass. There are kNodeWGPerCU runs per CU, so the
+// hardware's dispatch of the next waiting workgroup to the first free CU evens out what the cost model misses.
+constexpr int kNodeThreads  = 512;
+constexpr int kNodeWaves    = kNodeThreads / 64;
+constexpr int kNodeWGPerCU  = 4;
+constexpr int kNodeWGMax    = 1024;  // workgroups a plan can describe
 constexpr int kStagedVerts = 4913;  // vertices of the largest staged node grid (17^3), 16 B each (8 BF16 channels)
 
 struct ProjGeom
@@ -456,31 +458,37 @@ __device__ __forceinline__ Sample ctx_sample(const ProjGeom& g, const ChunkCtx& 
     auto a_of         = [&](int32_t jj) { return g.jitter ? jitter_a_h(hr, rank, jj) : 0.5f; };
     float tt;
     const uint32_t kind = c.kind[t];
-    if (kind != 0)
+    // one position formula for both kinds (sample_t is the uniform cell's t1 (1 - a) + t2 a); the kinds differ in
+    // the cell ends and the weight, so the lanes of a wave stay together through the hash and the position
+    float t1, t2;
+    if (kind == 2)
     {
-        float t1, t2;
-        if (kind == 1)
-        {
-            // boundaries 1..7 precomputed by the plan (proposal_cell of the same CDF row)
-            t1 = j == 0 ? tmin : c.cell[j - 1][t];
-            t2 = j + 1 >= n ? tmax : c.cell[j][t];
-        }
-        else
-        {
-            float cdf[8];
+        float cdf[8];
 #pragma unroll
-            for (int b = 0; b < 8; ++b) cdf[b] = c.cell[b][t];
-            t1 = proposal_cell(cdf, tmin, tmax, n, j);
-            t2 = proposal_cell(cdf, tmin, tmax, n, j + 1);
-        }
-        const float a  = a_of(j);
-        tt             = t1 * (1.0f - a) + t2 * a;
-        s.w            = t2 - t1;
+        for (int b = 0; b < 8; ++b) cdf[b] = c.cell[b][t];
+        t1 = proposal_cell(cdf, tmin, tmax, n, j);
+        t2 = proposal_cell(cdf, tmin, tmax, n, j + 1);
+    }
+    else if (kind == 1)
+    {
+        // boundaries 1..7 precomputed by the plan (proposal_cell of the same CDF row)
+        t1 = j == 0 ? tmin : c.cell[j - 1][t];
+        t2 = j + 1 >= n ? tmax : c.cell[j][t];
     }
     else
     {
-        // sample_t / sample_w with the ray's half of the hash taken from the context
-        tt             = sample_t(tmin, step, j, a_of(j));
+        t1 = tmin + (float)j * step;
+        t2 = tmin + (float)(j + 1) * step;
+    }
+    const float a = a_of(j);
+    tt            = t1 * (1.0f - a) + t2 * a;
+    if (kind != 0)
+    {
+        s.w = t2 - t1;
+    }
+    else
+    {
+        // midpoint-cell weight from the neighbours' positions (sample_w), their hashes taken only here
         const float lo = (j == 0) ? tmin : (sample_t(tmin, step, j - 1, a_of(j - 1)) + tt) * 0.5f;
         const float hi = (j == n - 1) ? tmax : (sample_t(tmin, step, j + 1, a_of(j + 1)) + tt) * 0.5f;
         s.w            = hi - lo;

@@ -208,11 +208,13 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
 #pragma unroll
                     for (int st = 0; st < 2; ++st) H1h[mt][st] = relu_bf16(bf16_frag(zh, st));
                 }
-                // hidden layer (h): output pre-activation, dz, relu(Z2) and D2 = dz [Z2 > 0]
+                // hidden layer (h): relu(Z2), the output pre-activation and dz, D2 = dz [Z2 > 0]. Here the output
+                // layer runs on the matrix cores too (row 0 of w3f against relu(Z2) in BF16): z only enters dz, where
+                // its BF16 rounding is of the size of the BF16 rounding of dz itself; the forward keeps it in FP32.
                 bf16x8 H2h[MT][2], D2h[MT][2];
                 float dz;
                 {
-                    float part = 0.f;
+                    f32x16 zacc = {};
 #pragma unroll
                     for (int mt = 0; mt < MT; ++mt)
                     {
@@ -224,18 +226,15 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
                             for (int st = 0; st < 2; ++st) acc = P::mma(w.w2[mt][ms][st][l], H1h[ms][st], acc);
                         }
 #pragma unroll
-                        for (int q = 0; q < 4; ++q)
+                        for (int st = 0; st < 2; ++st)
                         {
-                            const float4 w3 = *reinterpret_cast<const float4*>(&w.w3[32 * mt + 8 * q + 4 * hg]);
-                            part = fmaf(w3.x, relu_f32(acc[4 * q + 0]), part);
-                            part = fmaf(w3.y, relu_f32(acc[4 * q + 1]), part);
-                            part = fmaf(w3.z, relu_f32(acc[4 * q + 2]), part);
-                            part = fmaf(w3.w, relu_f32(acc[4 * q + 3]), part);
+                            H2h[mt][st] = relu_bf16(bf16_frag(acc, st));
+                            zacc        = P::mma(w.w3f[mt][st][l], H2h[mt][st], zacc);
                         }
-#pragma unroll
-                        for (int st = 0; st < 2; ++st) H2h[mt][st] = relu_bf16(bf16_frag(acc, st));
                     }
-                    const float z = part + __shfl_xor(part, 32) + w.b3;
+                    // row 0 of the product sits in register 0 of the lane group 0; group 1 takes it from there
+                    const float z0 = zacc[0];
+                    const float z  = (hg == 0 ? z0 : __shfl_xor(z0, 32)) + w.b3;
                     dz = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, z, g.beta) : 0.f;
                     bf16x8 dz8;
 #pragma unroll

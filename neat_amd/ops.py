@@ -258,6 +258,15 @@ class Projector:
              ptr(self.ws), self.ws.numel())
         return values, off
 
+    def ray_grad(self):
+        """(dL/dorigin, dL/ddirection) [R, 3] each of the last backward's loss, the samples' t and weights fixed
+        (neat_projector_ray_grad; call after backward or backward_stage(1))."""
+        go = torch.empty(self.R, 3, dtype=_F32, device=self.grid.device)
+        gd = torch.empty(self.R, 3, dtype=_F32, device=self.grid.device)
+        call("neat_projector_ray_grad", stream_ptr(self.stream), ctypes.byref(self.args), ptr(go), ptr(gd),
+             ptr(self.ws), self.ws.numel())
+        return go, gd
+
     def backward(self, grad_proj):
         grad_proj = grad_proj.contiguous()
         gg = torch.empty_like(self.grid)

@@ -699,6 +699,60 @@ void projector_bwd(const SampleCtx<T>& c, const SegList& s, const GridDesc& g, c
     }
 }
 
+// Gradient of L = sum_r grad_proj[r] * proj[r] with respect to the rays' origins and directions, the samples' t and
+// weights held fixed: the chain the reference runs when the rays require grad -- the global coordinate o + d t
+// recomputed with torch ops (HyperTree.cu:426-432), ComputeLocalSamples' backward (grad * 2) / (max - min)
+// (HyperTree.cu:116-199), the grid sample's uv gradient (IndirectGridSample3D.cu:752-790) of the decoder's
+// dL/dfeature, and the index_select backward onto the rays (sums over each ray's samples).
+template <typename T, typename G>
+void projector_ray_grad(const SampleCtx<T>& c, const SegList& s, const GridDesc& g, const G* grid, const MLPDesc& m,
+                        const float* params, int64_t num_rays, const T* grad_proj, T* grad_o, T* grad_d, int nthreads)
+{
+    const int64_t P = m.nparams();
+#pragma omp parallel num_threads(std::max(1, nthreads))
+    {
+        std::vector<T> gpar(P);  // the decoder's parameter gradient, not used here
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t r = 0; r < num_rays; ++r)
+        {
+            T go[3] = {0, 0, 0}, gd[3] = {0, 0, 0};
+            for (int64_t k = s.ray_seg_offset[r]; k < s.ray_seg_offset[r + 1]; ++k)
+            {
+                T tmin, tmax, step;
+                int32_t n, rank;
+                int64_t ray;
+                segment_params<T>(c, s, k, tmin, tmax, step, n, ray, rank);
+                int64_t node = s.seg_node[k];
+                int64_t slot = c.prefix[node];
+                const float* cdf = (s.seg_cdf && s.seg_cdf[8 * k + 7] > 0) ? s.seg_cdf + 8 * k : nullptr;
+                for (int32_t j = 0; j < n; ++j)
+                {
+                    T t, w;
+                    sample_tw<T>(c, cdf, tmin, tmax, step, n, j, ray, rank, t, w);
+                    T uv[3], feat[64], gfeat[64], guv[3];
+                    sample_uv<T>(c, ray, node, t, uv);
+                    grid_sample_one<T, G>(g, grid, slot, uv[0], uv[1], uv[2], feat);
+                    T gy[4] = {grad_proj[r] * w, 0, 0, 0};
+                    mlp_bwd<T, float>(m, params, feat, gy, gpar.data(), gfeat);
+                    grid_sample_bwd_one<T, G>(g, grid, slot, uv[0], uv[1], uv[2], gfeat, (T*)nullptr, guv);
+                    for (int i = 0; i < 3; ++i)
+                    {
+                        const T size = (T)c.pos_max[3 * node + i] - (T)c.pos_min[3 * node + i];
+                        const T gpos = (guv[i] * 2) / size;
+                        go[i] += gpos;
+                        gd[i] += gpos * t;
+                    }
+                }
+            }
+            for (int i = 0; i < 3; ++i)
+            {
+                grad_o[3 * r + i] = go[i];
+                grad_d[3 * r + i] = gd[i];
+            }
+        }
+    }
+}
+
 GridDesc make_grid(int64_t N, int32_t C, int32_t ID, int32_t IH, int32_t IW, const int64_t* strides)
 {
     GridDesc g;
@@ -1033,7 +1087,8 @@ extern "C"
         }
         else
         {
-            SampleCtx<double> c{pos_min, pos_max, prefix, ray_o, ray_d, jitter, seed, true};
+            // prec 1: tmin / tmax re-derived by the FP64 slab test; prec 2: the segments' FP32 t values as given
+            SampleCtx<double> c{pos_min, pos_max, prefix, ray_o, ray_d, jitter, seed, prec == 1};
             projector_fwd<double, float>(c, s, g, grid, m, params, num_rays, proj, nthreads);
         }
     }
@@ -1070,6 +1125,42 @@ extern "C"
             SampleCtx<double> c{pos_min, pos_max, prefix, ray_o, ray_d, jitter, seed, true};
             projector_bwd<double, float>(c, s, g, grid, m, params, num_rays, grad_proj, grad_grid, grid_numel,
                                          grad_params, nthreads);
+        }
+    }
+
+    // prec 0: the FP32 restatement; prec 1: FP64 with the samples' t taken from the given segments (the quantities
+    // the reference's positional gradient holds fixed)
+    void oracle_projector_ray_grad(int32_t prec, const float* pos_min, const float* pos_max, const int64_t* prefix,
+                                   const float* ray_o, const float* ray_d, int64_t num_rays,
+                                   const int64_t* ray_seg_offset, int64_t num_segments, const int32_t* seg_node,
+                                   const int32_t* seg_ray, const int32_t* seg_count, const float* seg_tmin,
+                                   const float* seg_tmax, const float* seg_cdf, int32_t jitter, uint64_t seed,
+                                   const float* grid, int64_t N, int32_t C, int32_t ID, int32_t IH, int32_t IW,
+                                   const int64_t* strides, int32_t in, int32_t hidden, int32_t layers, int32_t out,
+                                   const float* params, int32_t act, int32_t last_act, float beta,
+                                   const double* grad_proj, double* grad_o, double* grad_d, int32_t nthreads)
+    {
+        SegList s{num_segments, seg_node, seg_ray, seg_count, seg_tmin, seg_tmax, ray_seg_offset, seg_cdf};
+        GridDesc g = make_grid(N, C, ID, IH, IW, strides);
+        MLPDesc m{in, hidden, layers, out, act, last_act, beta};
+        if (prec == 0)
+        {
+            SampleCtx<float> c{pos_min, pos_max, prefix, ray_o, ray_d, jitter, seed, false};
+            std::vector<float> gp(num_rays), go(3 * num_rays), gd(3 * num_rays);
+            for (int64_t r = 0; r < num_rays; ++r) gp[r] = (float)grad_proj[r];
+            projector_ray_grad<float, float>(c, s, g, grid, m, params, num_rays, gp.data(), go.data(), gd.data(),
+                                             nthreads);
+            for (int64_t i = 0; i < 3 * num_rays; ++i)
+            {
+                grad_o[i] = go[i];
+                grad_d[i] = gd[i];
+            }
+        }
+        else
+        {
+            SampleCtx<double> c{pos_min, pos_max, prefix, ray_o, ray_d, jitter, seed, false};
+            projector_ray_grad<double, float>(c, s, g, grid, m, params, num_rays, grad_proj, grad_o, grad_d,
+                                              nthreads);
         }
     }
 

@@ -60,6 +60,9 @@ def lib():
             "oracle_projector_bwd": (None, [_I32, _P, _P, _P, _P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _I32,
                                             _U64, _P, _I64, _I32, _I32, _I32, _I32, _P, _I64, _I32, _I32, _I32,
                                             _I32, _P, _I32, _I32, _F, _P, _P, _P, _I32]),
+            "oracle_projector_ray_grad": (None, [_I32, _P, _P, _P, _P, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P,
+                                                 _I32, _U64, _P, _I64, _I32, _I32, _I32, _I32, _P, _I32, _I32, _I32,
+                                                 _I32, _P, _I32, _I32, _F, _P, _P, _P, _I32]),
             "oracle_max_threads": (_I32, []),
         }
         for name, (res, args) in sig.items():
@@ -256,7 +259,8 @@ def _proj_common(prec, tree, prefix, origin, direction, segs, grid, mlp, jitter,
 
 
 def projector_fwd(tree, prefix, origin, direction, segs, grid, mlp, jitter=False, seed=0, prec=0, nthreads=0):
-    """Per-ray projection (float64 array). prec 0: FP32 restatement, 1: FP64 oracle (structure frozen)."""
+    """Per-ray projection (float64 array). prec 0: FP32 restatement, 1: FP64 oracle (structure frozen, t re-derived
+    by the FP64 slab test), 2: FP64 on the segments' own t values."""
     keep, args = _proj_common(prec, tree, prefix, origin, direction, segs, grid, mlp, jitter, seed)
     R = keep["o"].shape[0]
     proj = np.empty(R, np.float64)
@@ -264,6 +268,21 @@ def projector_fwd(tree, prefix, origin, direction, segs, grid, mlp, jitter=False
                                ACT[mlp["act"]], ACT[mlp["last_act"]], mlp["beta"], _p(proj),
                                nthreads or max_threads())
     return proj
+
+
+def projector_ray_grad(tree, prefix, origin, direction, segs, grid, mlp, grad_proj, jitter=False, seed=0, prec=0,
+                       nthreads=0):
+    """Gradients of sum(grad_proj * proj) w.r.t. the ray origins and directions [R, 3] each, the samples' t and
+    weights held fixed (prec 0: FP32 restatement, 1: FP64 on the segments' t values)."""
+    keep, args = _proj_common(prec, tree, prefix, origin, direction, segs, grid, mlp, jitter, seed)
+    R = keep["o"].shape[0]
+    gp = _c(grad_proj, np.float64)
+    go = np.zeros((R, 3), np.float64)
+    gd = np.zeros((R, 3), np.float64)
+    lib().oracle_projector_ray_grad(*args, mlp["in"], mlp["hidden"], mlp["layers"], mlp["out"], _p(keep["params"]),
+                                    ACT[mlp["act"]], ACT[mlp["last_act"]], mlp["beta"], _p(gp), _p(go), _p(gd),
+                                    nthreads or max_threads())
+    return go, gd
 
 
 def projector_bwd(tree, prefix, origin, direction, segs, grid, mlp, grad_proj, jitter=False, seed=0, prec=0,

@@ -1,7 +1,8 @@
 """GPU checks of the ray generation (camera.hip and the module layer's CameraPoseModule / CameraModelModule / GetRays)
 against oracle/camera.py: the FP64 rotation, its gradients and the tangent update within 1e-12; the rays within
 1e-6; the pose gradients through GetRays against the oracle's closed form; and the pose gradient reaching the
-translation through the projector (rays that require grad take the kernel chain with the positional gradient)."""
+translation through the projector (rays that require grad keep the fused projector, whose backward returns the
+positional gradient; the kernel chain gives the same ray gradients)."""
 import numpy as np
 import pytest
 import torch
@@ -84,12 +85,13 @@ def test_get_rays_module_vs_oracle(H, align):
 
 
 def test_pose_gradient_through_projector(H):
-    """Rays from GetRays with pose gradients: ComputeImage takes the kernel chain (positional gradient) and the
-    translation gradient equals the per-image sum of the ray-origin gradients that the chain returns."""
+    """Rays from GetRays with pose gradients: ComputeImage keeps the fused projector, its ray gradients equal the
+    kernel chain's (use_fused_projector off) within 1e-5, and the translation gradient equals the per-image sum of the
+    ray-origin gradients."""
     print("case: pose gradient through the projector", flush=True)
     from test_gpu_host import make_geometry
     cfg = W.C1
-    sc, tree, _, geo = make_geometry(H, cfg, seed=2)
+    sc, tree, params, geo = make_geometry(H, cfg, seed=2)
     q, _ = poses(3, 5)
     t = np.array([[0.0, 0.0, -2.5], [0.1, 0.0, -2.4], [0.0, -0.1, -2.6]])
     # point the cameras at the origin: rotate e_z onto -t
@@ -121,6 +123,18 @@ def test_pose_gradient_through_projector(H):
     project(rays.origin, rays.direction).backward()
     project(o_leaf, d_leaf).backward()
     go = o_leaf.grad.double().cpu().numpy()
+    gdir = d_leaf.grad.double().cpu().numpy()
+    # the kernel chain (grid-sample backward with d uv, local-coordinate backward, index_select backward)
+    params.train_params.use_fused_projector = False
+    o2 = rays.origin.detach().clone().requires_grad_(True)
+    d2 = rays.direction.detach().clone().requires_grad_(True)
+    project(o2, d2).backward()
+    params.train_params.use_fused_projector = True
+    gc_o, gc_d = o2.grad.double().cpu().numpy(), d2.grad.double().cpu().numpy()
+    eo = np.linalg.norm(go - gc_o) / np.linalg.norm(gc_o)
+    ed = np.linalg.norm(gdir - gc_d) / np.linalg.norm(gc_d)
+    print("fused vs chain ray gradients: origin %.3e, direction %.3e" % (eo, ed), flush=True)
+    assert eo <= 1e-5 and ed <= 1e-5
     want = np.zeros((3, 3))
     np.add.at(want, img.cpu().numpy(), go)
     got = pose.translation.grad.cpu().numpy()

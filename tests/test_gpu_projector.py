@@ -113,3 +113,56 @@ def test_fused_projector_deterministic_forward_and_empty():
     segs0 = pipeline.sample(ds, far_o, far_d, emit=False)
     proj, gg, gp = run_fused(ds, far_o, far_d, segs0, False, 0, torch.ones(16, device=DEV))
     assert torch.all(proj == 0) and torch.all(gg == 0) and torch.all(gp == 0)
+
+
+@pytest.mark.parametrize("cfg_name,precision", [("C1", "fp32"), ("C2", "fp32"), ("C2", "bf16")])
+def test_ray_gradient_vs_oracle(oracle, cfg_name, precision):
+    """The fused projector's positional gradient (neat_projector_ray_grad: dL/dorigin, dL/ddirection with the samples'
+    t and weights fixed, as the reference's autograd holds them) against the oracle's restatement of the reference
+    chain: FP32 within 1e-5 of the FP32 oracle; every form's error against the FP64 oracle printed, BF16 within 5e-2
+    of it; and on C1 against central differences of the FP64 oracle projection (8 entries)."""
+    print("case: fused ray gradient", cfg_name, precision, flush=True)
+    cfg = getattr(W, cfg_name)
+    sc, ds, o, d, to, td = setup(cfg, 1024 if cfg_name == "C2" else 2048)
+    segs = pipeline.sample(ds, to, td, jitter=True, seed=6, emit=False)
+    g = np.random.default_rng(8).normal(size=o.shape[0]).astype(np.float32)
+    pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, to, td, segs, ds.grid, ds.params, ds.mlp, True, 6,
+                       precision=precision)
+    pj.forward()
+    pj.backward(torch.from_numpy(g).to(DEV))
+    go, gd = [t.cpu().numpy() for t in pj.ray_grad()]
+    hseg = {k: v.cpu().numpy() for k, v in segs.items() if torch.is_tensor(v)}
+    mlp = dict(sc["mlp"], params=sc["params"])
+    ro32, rd32 = oracle.projector_ray_grad(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, g, jitter=True,
+                                           seed=6, prec=0)
+    ro64, rd64 = oracle.projector_ray_grad(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, g, jitter=True,
+                                           seed=6, prec=1)
+    e32 = (rel(go, ro32), rel(gd, rd32))
+    e64 = (rel(go, ro64), rel(gd, rd64))
+    print("ray gradient vs fp32 oracle (o, d)", e32, "vs fp64 oracle", e64, flush=True)
+    if precision == "fp32":
+        assert max(e32) <= 1e-5
+    else:
+        assert max(e64) <= 5e-2
+    if cfg_name == "C1" and precision == "fp32":
+        def L(oo, dd):
+            p = oracle.projector_fwd(sc["tree"], sc["prefix"], oo, dd, hseg, sc["grid"], mlp, jitter=True, seed=6,
+                                     prec=2)
+            return float(np.dot(g.astype(np.float64), p))
+        rng = np.random.default_rng(9)
+        hit = np.flatnonzero(np.diff(hseg["ray_seg_offset"]) > 0)
+        fd, an = [], []
+        for r in rng.choice(hit, 4, replace=False):
+            for arr, grad, is_o in ((o, go, True), (d, gd, False)):
+                i = int(rng.integers(0, 3))
+                ap, am = arr.copy(), arr.copy()
+                ap[r, i] += np.float32(1e-5)
+                am[r, i] -= np.float32(1e-5)
+                h = float(ap[r, i]) - float(am[r, i])
+                lp = L(ap, d) if is_o else L(o, ap)
+                lm = L(am, d) if is_o else L(o, am)
+                fd.append((lp - lm) / h)
+                an.append(grad[r, i])
+        efd = rel(an, fd)
+        print("ray gradient vs central differences of the fp64 oracle (8 entries): rel", efd, flush=True)
+        assert efd <= 5e-3

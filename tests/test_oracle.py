@@ -473,3 +473,50 @@ def test_projector_gradient_matches_central_differences(oracle):
         h = float(ppl[i]) - float(pmi[i])
         fd = (loss(grid, ppl) - loss(grid, pmi)) / h
         assert abs(fd - gp[i]) <= 2e-3 * max(1e-2, abs(fd)), (i, fd, gp[i])
+
+
+def test_ray_gradient_matches_central_differences(oracle):
+    from neat_amd import workloads as W
+    """The oracle's positional gradient (the chain the reference runs for rays that require grad: o + d t, local
+    coordinates, the grid sample's uv gradient, index_select backward) against central differences of the FP64
+    projector on the same segments (their t and weights held fixed, as the reference's autograd holds them), for the
+    origin and the direction of 12 rays; and the FP32 restatement against the FP64 one."""
+    cfg = W.C1
+    sc = W.make_scene(cfg)
+    o, d = W.batch_rays(cfg, 256, seed=21)
+    segs = oracle.raysample(sc["tree"], sc["active_ids"], o, d, 16, emit=False, nthreads=1)
+    hit = np.flatnonzero(np.diff(segs["ray_seg_offset"]) > 0)
+    mlp = dict(sc["mlp"], params=sc["params"])
+    g = np.random.default_rng(3).normal(size=o.shape[0])
+    go, gd = oracle.projector_ray_grad(sc["tree"], sc["prefix"], o, d, segs, sc["grid"], mlp, g, jitter=True, seed=4,
+                                       prec=1, nthreads=1)
+    go32, gd32 = oracle.projector_ray_grad(sc["tree"], sc["prefix"], o, d, segs, sc["grid"], mlp, g, jitter=True,
+                                           seed=4, prec=0, nthreads=1)
+    assert np.abs(go[hit]).max() > 0
+    assert np.linalg.norm(go32 - go) <= 1e-5 * np.linalg.norm(go)
+    assert np.linalg.norm(gd32 - gd) <= 1e-5 * np.linalg.norm(gd)
+    assert np.all(go[np.setdiff1d(np.arange(o.shape[0]), hit)] == 0)
+
+    def L(oo, dd):
+        p = oracle.projector_fwd(sc["tree"], sc["prefix"], oo, dd, segs, sc["grid"], mlp, jitter=True, seed=4, prec=2,
+                                 nthreads=1)
+        return float(np.dot(g, p))
+
+    rng = np.random.default_rng(5)
+    fd, an = [], []
+    for r in rng.choice(hit, 12, replace=False):
+        for which, arr, grad in (("o", o, go), ("d", d, gd)):
+            i = int(rng.integers(0, 3))
+            h = np.float32(1e-5)  # small: the projection is only piecewise smooth (relu, trilinear cells)
+            ap, am = arr.copy(), arr.copy()
+            ap[r, i] += h
+            am[r, i] -= h
+            step = float(ap[r, i]) - float(am[r, i])
+            lp = L(ap, d) if which == "o" else L(o, ap)
+            lm = L(am, d) if which == "o" else L(o, am)
+            fd.append((lp - lm) / step)
+            an.append(grad[r, i])
+    fd, an = np.array(fd), np.array(an)
+    err = np.linalg.norm(fd - an) / np.linalg.norm(fd)
+    print("ray gradient vs central differences: rel", err)
+    assert err <= 5e-3
