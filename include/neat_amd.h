@@ -98,6 +98,16 @@ int neat_raysample_count(neat_stream_t stream, const float* node_pos_min, const 
                          int64_t* ray_seg_offset, int64_t* ray_sample_offset, int32_t* max_samples_per_ray_out,
                          void* workspace, size_t workspace_bytes);
 
+/* No counterpart in the reference: neat_raysample_count without a workspace, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The scan's tile sums live in
+ * a static device array of 131 KiB per GPU, which limits num_rays to 2^24 (NEAT_E_OVERFLOW beyond); it allocates
+ * nothing. Calls that overlap on one GPU share that array and race: serialize them (one stream, or events). */
+int neat_raysample_count_noalloc(neat_stream_t stream, const float* node_pos_min, const float* node_pos_max,
+                                 const uint8_t* node_flags, int32_t num_nodes, int32_t max_depth,
+                                 const float* ray_origin, const float* ray_direction, int64_t num_rays,
+                                 int32_t max_samples_per_node, int64_t* ray_seg_offset, int64_t* ray_sample_offset,
+                                 int32_t* max_samples_per_ray_out);
+
 /* Pass 2: the canonical segment list [num_segments]: node id, ray, sample count, tmin, tmax and the global
  * offset of the segment's first sample. */
 int neat_raysample_segments(neat_stream_t stream, const float* node_pos_min, const float* node_pos_max,

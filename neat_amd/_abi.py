@@ -34,6 +34,7 @@ SIGNATURES = {
     "neat_raysample_count": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P, _P, _I64, _I32, _P, _P, _P, _P, _SZ]),
     "neat_raysample_segments": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P, _P, _I64, _I32, _P, _P, _I64,
                                                _P, _P, _P, _P, _P, _P]),
+    "neat_raysample_count_noalloc": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P, _P, _I64, _I32, _P, _P, _P]),
     "neat_raysample_scratch_bytes": (_SZ, [_I64, _I32]),
     "neat_raysample_walk": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P, _P, _I64, _I32, _P, _P, _P, _P, _SZ, _P,
                                            _SZ]),

@@ -11,6 +11,7 @@
 // Output order is canonical (segments of a ray sorted by (tmin, node id)); the whole file is compiled without
 // floating-point contraction so counts and t values are bit-identical to the oracle.
 #include <algorithm>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -395,6 +396,36 @@ static RayScratch scratch_view(void* p, int64_t R, int32_t max_depth)
     return sc;
 }
 
+// ---- zero-allocation count pass ----------------------------------------------------------------------------------
+// The scan of the count pass needs (rays / 1024 + 1) int64 tile sums. neat_raysample_count_noalloc keeps them in
+// `scratch`, a file-scope device array sized for kNoAllocMaxRays rays (131 KiB; every GPU the library runs on holds
+// one copy until the process ends). Host code reaches it only through the address hipGetSymbolAddress returns for
+// the current device, fetched once per device and kept in scratch_addr.
+constexpr int64_t kNoAllocMaxRays = (int64_t)1 << 24;
+static __device__ __align__(256) int64_t scratch[kNoAllocMaxRays / 1024 + 2];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_addr[kMaxDevices] = {};
+
+// this device's copy of `scratch` (NEAT_OK), or an error status
+static int scratch_addr(void** out)
+{
+    int dev = 0;
+    NEAT_HIP(hipGetDevice(&dev));
+    NEAT_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d outside the %d this library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_addr[dev])
+    {
+        void* p = nullptr;
+        NEAT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        NEAT_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the count-pass scratch");
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return NEAT_OK;
+}
+
 extern "C"
 {
     size_t neat_raysample_workspace_bytes(int64_t num_rays) { return scan_workspace_bytes(num_rays); }
@@ -484,6 +515,24 @@ extern "C"
         st = exclusive_scan_i64(s, ray_seg_offset, R, workspace, workspace_bytes);
         if (st) return st;
         return exclusive_scan_i64(s, ray_sample_offset, R, workspace, workspace_bytes);
+    }
+
+    int neat_raysample_count_noalloc(neat_stream_t stream, const float* pmin, const float* pmax,
+                                     const uint8_t* flags, int32_t num_nodes, int32_t max_depth, const float* ro,
+                                     const float* rd, int64_t R, int32_t max_spn, int64_t* ray_seg_offset,
+                                     int64_t* ray_sample_offset, int32_t* max_per_ray)
+    {
+        if (R > kNoAllocMaxRays)
+            return fail(NEAT_E_OVERFLOW,
+                        "neat_raysample_count_noalloc: %lld rays exceed the %lld its scan scratch holds; call "
+                        "neat_raysample_count with a workspace",
+                        (long long)R, (long long)kNoAllocMaxRays);
+        void* ptr = nullptr;
+        int st    = scratch_addr(&ptr);
+        if (st) return st;
+        // the scan writes every tile sum it reads: nothing in the scratch needs clearing between calls
+        return neat_raysample_count(stream, pmin, pmax, flags, num_nodes, max_depth, ro, rd, R, max_spn,
+                                    ray_seg_offset, ray_sample_offset, max_per_ray, ptr, sizeof(scratch));
     }
 
     int neat_raysample_segments(neat_stream_t stream, const float* pmin, const float* pmax, const uint8_t* flags,
