@@ -45,6 +45,14 @@ def parse():
                          "of the batch's rays; the projection error is reported against the FP64 oracle with uniform "
                          "samples. uniform (the default otherwise): the reference's uniform in-node sampling over a "
                          "random N(0, 0.1) grid.")
+    ap.add_argument("--field", default="trained", choices=["trained", "phantom"],
+                    help="field of the proposal form: trained (default): the phantom fitted on the leaf grids, then "
+                         "--field-steps Adam steps of the fused BF16 training loop against the phantom's projections "
+                         "(untimed set-up); phantom: the fitted phantom alone")
+    ap.add_argument("--field-init", default="phantom", choices=["phantom", "random"],
+                    help="start of the trained field: the fitted phantom, or the random N(0, 0.1) grid and Xavier "
+                         "decoder of the configuration")
+    ap.add_argument("--field-steps", type=int, default=300, help="training steps of the trained field (2^18 rays)")
     ap.add_argument("--proposal-tau", type=float, default=0.3,
                     help="fallback threshold: segments in slots whose observed field variation exceeds it keep their "
                          "uniform samples")
@@ -228,16 +236,30 @@ def main():
     cfg = W.C2
     dev = "cuda"
     scene = W.make_scene(cfg, seed=a.seed)
+    field_info = "random grid N(0, 0.1)"
     if a.proposal:
         if a.path != "fused" or a.precision != "bf16":
             raise SystemExit("--sampling proposal needs the fused BF16 path")
-        scene["grid"], scene["params"] = W.phantom_field(scene, cfg)
+        if a.field == "phantom" or a.field_init == "phantom":
+            scene["grid"], scene["params"] = W.phantom_field(scene, cfg, device=dev)
+            field_info = "phantom fitted on the leaf grids"
     ds = pipeline.to_device(scene, cfg, dev)
+    if a.proposal and a.field == "trained":
+        t0 = time.time()
+        losses, _ = pipeline.train_field(ds, scene["phantom"], cfg, a.field_steps, cfg.batch, seed=a.seed + 31)
+        # the oracle (projection error) reads the trained field
+        scene["grid"] = ds.grid.cpu().contiguous().numpy()
+        scene["params"] = ds.params.cpu().numpy()
+        field_info = {"field": f"trained: {field_info if a.field_init == 'phantom' else 'random init'}, then "
+                               f"{a.field_steps} Adam steps (lr 1e-2 grid, 1e-3 decoder) of the fused BF16 path on "
+                               "2^18 rays against the phantom's projections",
+                      "loss_first": float(losses[0]), "loss_last10_mean": float(losses[-10:].mean()),
+                      "setup_s": round(time.time() - t0, 1)}
     batches = []
     for b in range(a.batches):
         o, d = W.batch_rays(cfg, cfg.batch, seed=a.seed + 100 + b)
-        tgt = scene["phantom"].line_integrals(o, d).astype(np.float32)
-        batches.append((torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev), torch.from_numpy(tgt).to(dev)))
+        to, td = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+        batches.append((to, td, scene["phantom"].line_integrals_torch(to, td).to(torch.float32)))
     torch.cuda.synchronize()
 
     # the BF16 grid copy is refreshed once per optimizer step in training; here the grid is fixed, so once
@@ -313,7 +335,7 @@ def main():
     if not a.no_compare and a.path == "fused" and a.precision == "bf16" and a.proposal and not a.regularizer:
         baselines = {}
         for name, path, prec in (("fp32_mfma", "fused", "fp32"), ("fp32_v1_fused_valu", "fused", "fp32_valu"),
-                                 ("fp32_v1_unfused", "unfused", "fp32")):
+                                 ("fp32_unfused_chain", "unfused", "fp32")):
             r = Runner(a, cfg, ds, batches, path, prec)
             ms, ns, _ = r.run(a.compare_steps, 1)
             baselines[name] = {"rays_per_s": round(cfg.batch / (ms / 1e3), 1), "ms_per_step": round(ms, 3),
@@ -327,8 +349,9 @@ def main():
         baselines["bf16_proposal_with_ray_grad"] = {"rays_per_s": round(cfg.batch / (ms / 1e3), 1),
                                                     "ms_per_step": round(ms, 3), "ray_grad_ms": round(sm["ray_grad"], 3)}
         del r
-        vs_baseline = baselines["fp32_v1_unfused"]["ratio"]
-        baselines["denominator"] = "fp32_v1_unfused (the first plain-FP32 GPU version: the unfused kernel chain)"
+        vs_baseline = baselines["fp32_v1_fused_valu"]["ratio"]
+        baselines["denominator"] = ("fp32_v1_fused_valu (the first plain-FP32 GPU projector: fused, vector ALUs, "
+                                    "kept unchanged)")
         # the training step: the headline form followed by Adam on the grid and the decoder and the BF16 refresh,
         # on copies of the field (the headline's tensors stay as they were)
         tds = pipeline.DeviceScene(ds.max_depth, ds.pos_min, ds.pos_max, ds.flags, ds.prefix, ds.grid.clone(),
@@ -352,7 +375,7 @@ def main():
               "path": a.path, "precision": prec_name, "batch_rays": cfg.batch,
               "samples_per_batch": round(mean_samples),
               "samples_per_s": round(mean_samples / (ms_per_step / 1e3), 1),
-              "field": "phantom fitted on the leaf grids" if prop is not None else "random grid N(0, 0.1)",
+              "field": field_info,
               "projection_error_vs_fp64": err,
               "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
               "build_id": {"libneat_amd": _abi.build_id(), "neat_host": host.load().build_id()}}

@@ -285,6 +285,233 @@ __device__ __forceinline__ typename P::Frag stage_frag(const float* buf, int row
     return f;
 }
 
+// Rows of the X staging tile: the 8 features, then the columns that turn the X product into the bias and output
+// gradients (E = dZ1 [x; 1]^T + dZ2 [e10]^T + h2 [dz e9]^T, with the B operand masked per product).
+constexpr int kXOnes1 = 8, kXDz = 9, kXOnes2 = 10, kXRows = 11;
+
+// The backward's [row][sample] staging buffers: BF16 (rows of 40) for the BF16 form, where the values are rounded
+// once when staged and read as ready operands, FP32 (rows of kRS) otherwise.
+template <typename P>
+struct Stage
+{
+    static constexpr int kStride = P::kBF16 ? 40 : kRS;  // elements per row; 16-byte rows either way
+    typedef typename std::conditional<P::kBF16, unsigned short, float>::type E;
+    // elements of one wave's buffers for hidden width h: dZ / activations (2h rows) and the X tile
+    static constexpr int elems(int h) { return (2 * h + kXRows) * kStride; }
+    __device__ static __forceinline__ void put(E* b, int row, int col, float v)
+    {
+        if constexpr (P::kBF16)
+            b[row * kStride + col] = (unsigned short)bf16_bits(v);
+        else
+            b[row * kStride + col] = v;
+    }
+    // A / B fragment over the tile's samples (see stage_frag); masked lanes read row 0 and discard it
+    __device__ static __forceinline__ typename P::Frag frag(const E* b, int row, int st, int hg, bool zero)
+    {
+        if constexpr (P::kBF16)
+        {
+            const uint4 u = *reinterpret_cast<const uint4*>(b + (zero ? 0 : row) * kStride + 16 * st + 8 * hg);
+            typename P::Frag f;
+            __builtin_memcpy(&f, &u, sizeof(f));
+            if (zero) f = typename P::Frag{};
+            return f;
+        }
+        else
+        {
+            return zero ? 0.f : b[row * kStride + 2 * st + hg];
+        }
+    }
+};
+
+// One wave's staging buffers: sA (H rows), sB (H rows), sX (kXRows rows). The constant rows of sX are set here.
+template <typename P, int H>
+struct BwdStage
+{
+    typedef typename Stage<P>::E E;
+    E* sA;
+    E* sB;
+    E* sX;
+    __device__ explicit BwdStage(E* base) : sA(base), sB(base + H * Stage<P>::kStride), sX(sB + H * Stage<P>::kStride)
+    {
+        const int l = threadIdx.x & 63;
+        if (l < 32)
+        {
+            Stage<P>::put(sX, kXOnes1, l, 1.f);
+            Stage<P>::put(sX, kXOnes2, l, 1.f);
+        }
+    }
+};
+
+// A wave's running parameter-gradient products: dW2 (hidden layer), E (first layer in columns 0..7, its bias in
+// column kXOnes1, the output-layer weights in column kXDz, the hidden bias in column kXOnes2), b3 (output bias).
+template <int MT>
+struct BwdAcc
+{
+    f32x16 dW2[MT][MT];
+    f32x16 E[MT];
+    float b3;
+    __device__ BwdAcc() : b3(0.f)
+    {
+#pragma unroll
+        for (int a = 0; a < MT; ++a)
+        {
+            E[a] = f32x16{};
+#pragma unroll
+            for (int b = 0; b < MT; ++b) dW2[a][b] = f32x16{};
+        }
+    }
+};
+
+// Backward of one tile of 32 samples on a wave: recompute the forward (tile_forward_mfma), take dz = gy *
+// last_act'(z) (0 for invalid lanes), back-propagate on the matrix cores, and add the tile's parameter-gradient
+// products to acc (products over the tile's samples, staged through LDS as [row][sample]). Returns dL/dx in
+// registers 0..3 of each lane (channels 4g..4g+3 of its sample, g = lane / 32).
+template <typename P, int H, int ACT>
+__device__ __forceinline__ f32x16 tile_backward_mfma(const WeightLDS<P, H>& w, const float (&x4)[4], float gy,
+                                                     bool valid, int last_act, float beta, const BwdStage<P, H>& sb,
+                                                     BwdAcc<H / 32>& acc)
+{
+    constexpr int MT = H / 32, S = steps_per_tile<P>();
+    constexpr int KSTEPS = 32 / (2 * P::KS);  // sample steps of a weight-gradient product (FP32 16, BF16 2)
+    typedef Stage<P> ST;
+    const int l = threadIdx.x & 63, n = l & 31, hg = l >> 5;
+    Fwd<MT> f;
+    tile_forward_mfma<P, H, ACT>(w, x4, beta, f);
+    const float dz = valid ? gy * act_d(last_act, f.z, beta) : 0.f;
+    if (hg == 0) acc.b3 += dz;
+    // X tile and the first staged pair: dZ2 (A) and act(z1) (B)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ST::put(sb.sX, 4 * hg + j, n, x4[j]);
+    if (hg == 0) ST::put(sb.sX, kXDz, n, dz);
+    f32x16 dZ2[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+    {
+#pragma unroll
+        for (int i2 = 0; i2 < 16; ++i2)
+        {
+            const int r = 32 * mt + crow(i2, hg);
+            dZ2[mt][i2] = w.vec[2][r] * dz * act_d(ACT, f.z2[mt][i2], beta);
+            ST::put(sb.sA, r, n, dZ2[mt][i2]);
+            ST::put(sb.sB, r, n, act_f(ACT, f.z1[mt][i2], beta));
+        }
+    }
+    wave_lds_fence();
+    // dW(hidden) += dZ2 act(z1)^T; E += dZ2 e10^T (the hidden bias gradient, column 10)
+#pragma unroll
+    for (int st = 0; st < KSTEPS; ++st)
+    {
+        const auto bx = ST::frag(sb.sX, n, st, hg, n != kXOnes2);
+        typename P::Frag bf[MT];
+#pragma unroll
+        for (int m1 = 0; m1 < MT; ++m1) bf[m1] = ST::frag(sb.sB, 32 * m1 + n, st, hg, false);
+#pragma unroll
+        for (int m2 = 0; m2 < MT; ++m2)
+        {
+            const auto a = ST::frag(sb.sA, 32 * m2 + n, st, hg, false);
+            acc.E[m2]    = P::mma(a, bx, acc.E[m2]);
+#pragma unroll
+            for (int m1 = 0; m1 < MT; ++m1) acc.dW2[m2][m1] = P::mma(a, bf[m1], acc.dW2[m2][m1]);
+        }
+    }
+    wave_lds_fence();
+    // E += act(z2) (dz e9)^T: the output-layer weight gradient, column 9
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+    {
+#pragma unroll
+        for (int i2 = 0; i2 < 16; ++i2) ST::put(sb.sB, 32 * mt + crow(i2, hg), n, act_f(ACT, f.z2[mt][i2], beta));
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int st = 0; st < KSTEPS; ++st)
+    {
+        const auto bx = ST::frag(sb.sX, n, st, hg, n != kXDz);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc.E[mt] = P::mma(ST::frag(sb.sB, 32 * mt + n, st, hg, false), bx, acc.E[mt]);
+    }
+    // dZ1 = (W1^T dZ2) * act'(z1), in registers
+    f32x16 dZ1[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+    {
+        f32x16 a = {};
+#pragma unroll
+        for (int ms = 0; ms < MT; ++ms)
+        {
+#pragma unroll
+            for (int st = 0; st < S; ++st)
+                a = P::mma(w.w2t[mt][ms][st][l], P::from_acc(dZ2[ms], st, [](float v, int) { return v; }), a);
+        }
+#pragma unroll
+        for (int i2 = 0; i2 < 16; ++i2) a[i2] *= act_d(ACT, f.z1[mt][i2], beta);
+        dZ1[mt] = a;
+    }
+    // dL/dx: rows 0..7 of W0^T dZ1, i.e. registers 0..3 of each lane group (channels 4g..4g+3)
+    f32x16 dx = {};
+#pragma unroll
+    for (int ms = 0; ms < MT; ++ms)
+    {
+#pragma unroll
+        for (int st = 0; st < S; ++st)
+            dx = P::mma(w.w1t[ms][st][l], P::from_acc(dZ1[ms], st, [](float v, int) { return v; }), dx);
+    }
+    // E += dZ1 [x; 1]^T: the first-layer weight gradient (columns 0..7) and bias gradient (column 8)
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+    {
+#pragma unroll
+        for (int i2 = 0; i2 < 16; ++i2) ST::put(sb.sA, 32 * mt + crow(i2, hg), n, dZ1[mt][i2]);
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int st = 0; st < KSTEPS; ++st)
+    {
+        const auto bx = ST::frag(sb.sX, n, st, hg, n > kXOnes1);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc.E[mt] = P::mma(ST::frag(sb.sA, 32 * mt + n, st, hg, false), bx, acc.E[mt]);
+    }
+    wave_lds_fence();
+    return dx;
+}
+
+// A wave's parameter-gradient products written to its slice `part` of the partials, in the packed layout
+// (C -> H -> H -> 1: W0, b0, W1, b1, W2, b2). Every entry of the slice is written.
+template <int H, int C>
+__device__ __forceinline__ void write_param_partials(BwdAcc<H / 32>& acc, float* part)
+{
+    constexpr int MT = H / 32;
+    const int l = threadIdx.x & 63, n = l & 31, hg = l >> 5;
+    const int oW0 = 0, ob0 = H * C, oW1 = ob0 + H, ob1 = oW1 + H * H, oW2 = ob1 + H, ob2 = oW2 + H;
+#pragma unroll
+    for (int m2 = 0; m2 < MT; ++m2)
+    {
+#pragma unroll
+        for (int m1 = 0; m1 < MT; ++m1)
+        {
+#pragma unroll
+            for (int i2 = 0; i2 < 16; ++i2) part[oW1 + (32 * m2 + crow(i2, hg)) * H + 32 * m1 + n] = acc.dW2[m2][m1][i2];
+        }
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+    {
+#pragma unroll
+        for (int i2 = 0; i2 < 16; ++i2)
+        {
+            const int r = 32 * mt + crow(i2, hg);
+            if (n < C) part[oW0 + r * C + n] = acc.E[mt][i2];
+            if (n == kXOnes1) part[ob0 + r] = acc.E[mt][i2];
+            if (n == kXDz) part[oW2 + r] = acc.E[mt][i2];
+            if (n == kXOnes2) part[ob1 + r] = acc.E[mt][i2];
+        }
+    }
+    float b3 = acc.b3;
+#pragma unroll
+    for (int off = 16; off >= 1; off >>= 1) b3 += __shfl_xor(b3, off);
+    if (l == 0) part[ob2] = b3;
+}
+
 // ---- BF16 relu decoders: weights for products in both orientations --------------------------------------------
 // The A and B fragments of v_mfma_f32_32x32x16_bf16 share one register layout (lane l%32 = row of A / column of B,
 // lane l/32 and slot j = k), so mma(a, b) and mma(b, a) give a product and its transpose from the same registers.

@@ -555,6 +555,135 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
         *reinterpret_cast<float4*>(dst + i) = *reinterpret_cast<const float4*>(sg + i);
 }
 
+// The BF16 form's grid-gradient scatter: all 8 channels in one pass, in 32-bit fixed point. A chunk of n samples
+// adds at most n |dL/dfeature| (trilinear weights are at most 1 and sum to 1 per sample) to any vertex, so with the
+// scale 2^F, F = 30 - ceil(log2 n) - e (|dL/dfeature| < 2^e over the slot's samples, dmax[1 + slot]), every vertex sum
+// and every rounding of its contributions stays inside int32. The quantum is 2^-F, below 2^-30 n max|dL/dfeature|:
+// with scatter chunks of at most 65536 samples, under 2^-14 of the slot's largest value. (F is clamped to +-100, far
+// outside what float gradients reach, so that 2^F stays finite.) Integer adds are exact and order-independent, so the
+// gradient stays bit-reproducible. Lanes are sample streams as in
+// k_proj_bw2; a lane keeps the current cell's 8 corners x 8 channels in registers, prescaled, and converts them once
+// per cell (round to nearest).
+__global__ void __launch_bounds__(kScatterThreads) k_proj_bw2_i32(ProjGeom g, Plan p, const float* dfeat,
+                                                                  const float4* sgc, const unsigned int* dmax,
+                                                                  float* grad_grid, float* partial_grids)
+{
+    extern __shared__ float sgf[];
+    int32_t* sg        = reinterpret_cast<int32_t*>(sgf);  // planar [channel][vertex]
+    const int64_t c    = blockIdx.x;
+    if (c >= p.chunk_off2[g.N]) return;
+    const int32_t slot = p.chunk2_slot[c];
+    const int32_t q0 = p.chunk2_q0[c], q1 = p.chunk2_q1[c];
+    const int32_t G = g.G, nv = G * G * G;
+    for (int32_t i = threadIdx.x * 4; i < 8 * nv; i += blockDim.x * 4)
+        *reinterpret_cast<int4*>(sg + i) = make_int4(0, 0, 0, 0);
+    __syncthreads();
+    const int64_t base = p.nm_soff[q0];
+    const int64_t ns   = p.nm_soff[q1] - base;
+    // the chunk's scale, from its slot's largest |dL/dfeature|; F stays inside float's exponent range
+    int e;
+    frexpf(__uint_as_float(dmax[1 + slot]), &e);
+    int lg = 0;
+    while (((int64_t)1 << lg) < max<int64_t>(ns, 1)) ++lg;
+    const int F       = min(100, max(-100, 30 - lg - e));
+    const float scale = ldexpf(1.0f, F);
+    const int64_t L2 = (ns + blockDim.x - 1) / blockDim.x;
+    const int64_t s2 = (int64_t)threadIdx.x * L2;
+    const int64_t e2 = min(ns, s2 + L2);
+    const float4* d0 = reinterpret_cast<const float4*>(dfeat) + base;          // channels 0..3
+    const float4* d1 = reinterpret_cast<const float4*>(dfeat) + g.S + base;    // channels 4..7
+    if (s2 < e2)
+    {
+        int cell = -1;
+        int cx0 = 0, cx1 = 0, cy0 = 0, cy1 = 0, cz0 = 0, cz1 = 0;
+        float acc[8][8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[k][q] = 0.f;
+        auto flush = [&]() {
+            const int vz[2] = {cz0 * G * G, cz1 * G * G}, vy[2] = {cy0 * G, cy1 * G}, vx[2] = {cx0, cx1};
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+            {
+                const int v = vz[k >> 2] + vy[(k >> 1) & 1] + vx[k & 1];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                {
+                    atomicAdd(sg + q * nv + v, (int32_t)__builtin_rintf(acc[k][q]));
+                    acc[k][q] = 0.f;
+                }
+            }
+        };
+        constexpr int kSB = 2;
+        for (int64_t s0 = s2; s0 < e2; s0 += kSB)
+        {
+            float4 gb[kSB], da[kSB], db[kSB];
+#pragma unroll
+            for (int b = 0; b < kSB; ++b)
+            {
+                if (s0 + b < e2)
+                {
+                    gb[b] = sgc[base + s0 + b];
+                    da[b] = d0[s0 + b];
+                    db[b] = d1[s0 + b];
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < kSB; ++b)
+            {
+                if (s0 + b >= e2) break;
+                const float4 gc = gb[b];
+                const int x0 = (int)floorf(gc.x), y0 = (int)floorf(gc.y), z0 = (int)floorf(gc.z);
+                const int cc = (x0 + 1) + (G + 2) * ((y0 + 1) + (G + 2) * (z0 + 1));
+                if (cc != cell)
+                {
+                    if (cell >= 0) flush();
+                    cell = cc;
+                    cx0 = clamp_g(x0, G);
+                    cx1 = clamp_g(x0 + 1, G);
+                    cy0 = clamp_g(y0, G);
+                    cy1 = clamp_g(y0 + 1, G);
+                    cz0 = clamp_g(z0, G);
+                    cz1 = clamp_g(z0 + 1, G);
+                }
+                const float fx1 = gc.x - (float)x0, fx0 = (float)(x0 + 1) - gc.x;
+                const float fy1 = gc.y - (float)y0, fy0 = (float)(y0 + 1) - gc.y;
+                const float fz1 = gc.z - (float)z0, fz0 = (float)(z0 + 1) - gc.z;
+                const float wxy[4] = {fx0 * fy0, fx1 * fy0, fx0 * fy1, fx1 * fy1};
+                const float dv[8] = {da[b].x * scale, da[b].y * scale, da[b].z * scale, da[b].w * scale,
+                                     db[b].x * scale, db[b].y * scale, db[b].z * scale, db[b].w * scale};
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                {
+                    const float wk = wxy[k & 3] * ((k & 4) ? fz1 : fz0);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) acc[k][q] = fmaf(wk, dv[q], acc[k][q]);
+                }
+            }
+        }
+        if (cell >= 0) flush();
+    }
+    __syncthreads();
+    const bool split = p.chunk_off2[slot + 1] - p.chunk_off2[slot] > 1;
+    float* dst       = split ? partial_grids + c * (int64_t)nv * 8 : grad_grid + (int64_t)slot * nv * 8;
+    const float inv  = ldexpf(1.0f, -F);
+    for (int32_t v = threadIdx.x; v < nv; v += blockDim.x)
+    {
+        float4 a, b;
+        a.x = (float)sg[0 * nv + v] * inv;
+        a.y = (float)sg[1 * nv + v] * inv;
+        a.z = (float)sg[2 * nv + v] * inv;
+        a.w = (float)sg[3 * nv + v] * inv;
+        b.x = (float)sg[4 * nv + v] * inv;
+        b.y = (float)sg[5 * nv + v] * inv;
+        b.z = (float)sg[6 * nv + v] * inv;
+        b.w = (float)sg[7 * nv + v] * inv;
+        *reinterpret_cast<float4*>(dst + (int64_t)v * 8)     = a;
+        *reinterpret_cast<float4*>(dst + (int64_t)v * 8 + 4) = b;
+    }
+}
+
 // Slots cut into several scatter chunks: sum the chunk grids in chunk order. Slots without samples: zero.
 __global__ void __launch_bounds__(256) k_proj_bw2_reduce(ProjGeom g, Plan p, int32_t gsize, const float* partial_grids,
                                                          float* grad_grid)
@@ -640,42 +769,6 @@ __global__ void __launch_bounds__(256, 2) k_mfma_fwd(ProjGeom g, Plan p, const f
     }
 }
 
-// The backward's [row][sample] staging buffers: BF16 (rows of 40) for the BF16 form, where the values are rounded
-// once when staged and read as ready operands, FP32 (rows of kRS) otherwise.
-template <typename P>
-struct Stage
-{
-    static constexpr int kStride = P::kBF16 ? 40 : kRS;  // elements per row; 16-byte rows either way
-    typedef typename std::conditional<P::kBF16, unsigned short, float>::type E;
-    __device__ static __forceinline__ void put(E* b, int row, int col, float v)
-    {
-        if constexpr (P::kBF16)
-            b[row * kStride + col] = (unsigned short)bf16_bits(v);
-        else
-            b[row * kStride + col] = v;
-    }
-    // A / B fragment over the tile's samples (see stage_frag); masked lanes read row 0 and discard it
-    __device__ static __forceinline__ typename P::Frag frag(const E* b, int row, int st, int hg, bool zero)
-    {
-        if constexpr (P::kBF16)
-        {
-            const uint4 u = *reinterpret_cast<const uint4*>(b + (zero ? 0 : row) * kStride + 16 * st + 8 * hg);
-            typename P::Frag f;
-            __builtin_memcpy(&f, &u, sizeof(f));
-            if (zero) f = typename P::Frag{};
-            return f;
-        }
-        else
-        {
-            return zero ? 0.f : b[row * kStride + 2 * st + hg];
-        }
-    }
-};
-
-// Rows of the X staging tile: the 8 features, then the columns that turn the X product into the bias and output
-// gradients (E = dZ1 [x; 1]^T + dZ2 [e10]^T + h2 [dz e9]^T, with the B operand masked per product).
-constexpr int kXOnes1 = 8, kXDz = 9, kXOnes2 = 10, kXRows = 11;
-
 // Backward: persistent workgroups of 4 waves; per wave and tile of 32 samples, recompute the forward, back-propagate
 // on the matrix cores, write dL/dfeature per sample (for the LDS scatter), and accumulate the parameter gradient
 // with products over the tile's samples (staged through LDS as [row][sample]). Each wave writes its own slice of the
@@ -685,35 +778,23 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
                                                   const void* grid_lowp, const float* grad_proj, float* dfeat,
                                                   float4* sgc, unsigned int* dmax, float* partials, int32_t P_total)
 {
-    constexpr int MT = H / 32, S = steps_per_tile<P>(), C = 8;
-    constexpr int KSTEPS = 32 / (2 * P::KS);  // sample steps of a weight-gradient product (FP32 16, BF16 2)
+    constexpr int MT = H / 32, C = 8;
     __shared__ WeightLDS<P, H> w;
     __shared__ ChunkCtx cx;
     typedef Stage<P> ST;
-    typedef typename ST::E SE;
-    __shared__ __align__(16) SE stage[4][(2 * H + kXRows) * ST::kStride];
+    __shared__ __align__(16) typename ST::E stage[4][ST::elems(H)];
     load_weights<P, H, C>(params, w);
-    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
-    SE* sA = stage[wave];
-    SE* sB = sA + H * ST::kStride;
-    SE* sX = sB + H * ST::kStride;
-    f32x16 dW2[MT][MT], E[MT];
-#pragma unroll
-    for (int a = 0; a < MT; ++a)
-    {
-        E[a] = f32x16{};
-#pragma unroll
-        for (int b = 0; b < MT; ++b) dW2[a][b] = f32x16{};
-    }
-    float b3 = 0.f;
+    const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, hg = l >> 5;
+    BwdStage<P, H> sb(stage[wave]);
+    BwdAcc<MT> acc;
     float amax = 0.f;  // largest |dL/dfeature| this lane wrote
-    // the constant rows of the X tile
-    if (hg == 0)
-    {
-        ST::put(sX, kXOnes1, n, 1.f);
-        ST::put(sX, kXOnes2, n, 1.f);
-    }
     const int64_t nchunks = p.chunk_off1[g.N];
+    // the chunk's largest |dL/dfeature| into its slot's entry (dmax[1 + slot])
+    auto slot_max = [&](float m, int32_t slot) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        if (l == 0) atomicMax(dmax + 1 + slot, __float_as_uint(m));
+    };
     for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x)
     {
         const int32_t slot = p.chunk1_slot[c];
@@ -727,6 +808,7 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
         const void* ngrid = node_grid_ptr(g, grid_lowp, P::kBF16, slot);
         __syncthreads();
         const int32_t nsamp = cx.beg[nseg];
+        float cmax = 0.f;  // this chunk's
         for (int32_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
         {
             const int32_t wn = min(kFwdChunk, nsamp - w0);
@@ -735,113 +817,13 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
             // pairs of tiles per wave (fetch_pair)
             auto tile = [&](int32_t i, const float (&x4)[4], float gy) {
                 const bool valid = i < wn;
-                Fwd<MT> f;
-                tile_forward_mfma<P, H, ACT>(w, x4, g.beta, f);
-                const float dz = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, f.z, g.beta) : 0.f;
-                if (hg == 0) b3 += dz;
-                // X tile and the first staged pair: dZ2 (A) and act(z1) (B)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) ST::put(sX, 4 * hg + j, n, x4[j]);
-                if (hg == 0) ST::put(sX, kXDz, n, dz);
-                f32x16 dZ2[MT];
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
+                const f32x16 dx = tile_backward_mfma<P, H, ACT>(w, x4, gy, valid, NEAT_ACT_SOFTPLUS, g.beta, sb, acc);
+                if (valid)
                 {
-#pragma unroll
-                    for (int i2 = 0; i2 < 16; ++i2)
-                    {
-                        const int r    = 32 * mt + crow(i2, hg);
-                        dZ2[mt][i2]    = w.vec[2][r] * dz * act_d(ACT, f.z2[mt][i2], g.beta);
-                        ST::put(sA, r, n, dZ2[mt][i2]);
-                        ST::put(sB, r, n, act_f(ACT, f.z1[mt][i2], g.beta));
-                    }
+                    *reinterpret_cast<float4*>(dfeat + ((int64_t)hg * g.S + base + w0 + i) * 4) =
+                        make_float4(dx[0], dx[1], dx[2], dx[3]);
+                    cmax = fmaxf(cmax, fmaxf(fmaxf(fabsf(dx[0]), fabsf(dx[1])), fmaxf(fabsf(dx[2]), fabsf(dx[3]))));
                 }
-                wave_lds_fence();
-                // dW(hidden) += dZ2 act(z1)^T; E += dZ2 e10^T (the hidden bias gradient, column 10)
-#pragma unroll
-                for (int st = 0; st < KSTEPS; ++st)
-                {
-                    const auto bx = ST::frag(sX, n, st, hg, n != kXOnes2);
-                    typename P::Frag bf[MT];
-#pragma unroll
-                    for (int m1 = 0; m1 < MT; ++m1) bf[m1] = ST::frag(sB, 32 * m1 + n, st, hg, false);
-#pragma unroll
-                    for (int m2 = 0; m2 < MT; ++m2)
-                    {
-                        const auto a = ST::frag(sA, 32 * m2 + n, st, hg, false);
-                        E[m2]        = P::mma(a, bx, E[m2]);
-#pragma unroll
-                        for (int m1 = 0; m1 < MT; ++m1) dW2[m2][m1] = P::mma(a, bf[m1], dW2[m2][m1]);
-                    }
-                }
-                wave_lds_fence();
-                // E += act(z2) (dz e9)^T: the output-layer weight gradient, column 9
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                {
-#pragma unroll
-                    for (int i2 = 0; i2 < 16; ++i2)
-                        ST::put(sB, 32 * mt + crow(i2, hg), n, act_f(ACT, f.z2[mt][i2], g.beta));
-                }
-                wave_lds_fence();
-#pragma unroll
-                for (int st = 0; st < KSTEPS; ++st)
-                {
-                    const auto bx = ST::frag(sX, n, st, hg, n != kXDz);
-#pragma unroll
-                    for (int mt = 0; mt < MT; ++mt) E[mt] = P::mma(ST::frag(sB, 32 * mt + n, st, hg, false), bx, E[mt]);
-                }
-                // dZ1 = (W1^T dZ2) * act'(z1), in registers
-                f32x16 dZ1[MT];
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                {
-                    f32x16 acc = {};
-#pragma unroll
-                    for (int ms = 0; ms < MT; ++ms)
-                    {
-#pragma unroll
-                        for (int st = 0; st < S; ++st)
-                            acc = P::mma(w.w2t[mt][ms][st][l], P::from_acc(dZ2[ms], st, [](float v, int) { return v; }),
-                                         acc);
-                    }
-#pragma unroll
-                    for (int i2 = 0; i2 < 16; ++i2) acc[i2] *= act_d(ACT, f.z1[mt][i2], g.beta);
-                    dZ1[mt] = acc;
-                }
-                // dL/dfeature: rows 0..7 of W0^T dZ1, i.e. registers 0..3 of each lane group (channels 4g..4g+3)
-                {
-                    f32x16 dx = {};
-#pragma unroll
-                    for (int ms = 0; ms < MT; ++ms)
-                    {
-#pragma unroll
-                        for (int st = 0; st < S; ++st)
-                            dx = P::mma(w.w1t[ms][st][l], P::from_acc(dZ1[ms], st, [](float v, int) { return v; }), dx);
-                    }
-                    if (valid)
-                    {
-                        *reinterpret_cast<float4*>(dfeat + ((int64_t)hg * g.S + base + w0 + i) * 4) =
-                            make_float4(dx[0], dx[1], dx[2], dx[3]);
-                        amax = fmaxf(amax, fmaxf(fmaxf(fabsf(dx[0]), fabsf(dx[1])), fmaxf(fabsf(dx[2]), fabsf(dx[3]))));
-                    }
-                }
-                // E += dZ1 [x; 1]^T: the first-layer weight gradient (columns 0..7) and bias gradient (column 8)
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                {
-#pragma unroll
-                    for (int i2 = 0; i2 < 16; ++i2) ST::put(sA, 32 * mt + crow(i2, hg), n, dZ1[mt][i2]);
-                }
-                wave_lds_fence();
-#pragma unroll
-                for (int st = 0; st < KSTEPS; ++st)
-                {
-                    const auto bx = ST::frag(sX, n, st, hg, n > kXOnes1);
-#pragma unroll
-                    for (int mt = 0; mt < MT; ++mt) E[mt] = P::mma(ST::frag(sA, 32 * mt + n, st, hg, false), bx, E[mt]);
-                }
-                wave_lds_fence();
             };
             for (int32_t p0 = wave * 64; p0 < wn; p0 += 256)
             {
@@ -855,41 +837,16 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
                     float xs[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) xs[j] = h ? cur.xb[j] : cur.xa[j];
-                    tile(p0 + 32 * h + n, xs, h ? cur.vb : cur.va);
+                    tile(p0 + 32 * h + (l & 31), xs, h ? cur.vb : cur.va);
                 }
             }
             __syncthreads();
         }
+        slot_max(cmax, slot);
+        amax = fmaxf(amax, cmax);
     }
-    // ---- this wave's slice of the partial parameter gradient (packed layout) ----
-    float* part   = partials + ((int64_t)blockIdx.x * 4 + wave) * P_total;
-    const int oW0 = 0, ob0 = H * C, oW1 = ob0 + H, ob1 = oW1 + H * H, oW2 = ob1 + H, ob2 = oW2 + H;
-#pragma unroll
-    for (int m2 = 0; m2 < MT; ++m2)
-    {
-#pragma unroll
-        for (int m1 = 0; m1 < MT; ++m1)
-        {
-#pragma unroll
-            for (int i2 = 0; i2 < 16; ++i2) part[oW1 + (32 * m2 + crow(i2, hg)) * H + 32 * m1 + n] = dW2[m2][m1][i2];
-        }
-    }
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-    {
-#pragma unroll
-        for (int i2 = 0; i2 < 16; ++i2)
-        {
-            const int r = 32 * mt + crow(i2, hg);
-            if (n < C) part[oW0 + r * C + n] = E[mt][i2];
-            if (n == kXOnes1) part[ob0 + r] = E[mt][i2];
-            if (n == kXDz) part[oW2 + r] = E[mt][i2];
-            if (n == kXOnes2) part[ob1 + r] = E[mt][i2];
-        }
-    }
-#pragma unroll
-    for (int off = 16; off >= 1; off >>= 1) b3 += __shfl_xor(b3, off);
-    if (l == 0) part[ob2] = b3;
+    // this wave's slice of the partial parameter gradient (packed layout)
+    write_param_partials<H, C>(acc, partials + ((int64_t)blockIdx.x * 4 + wave) * P_total);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off));
     if (l == 0) atomicMax(dmax, __float_as_uint(amax));  // non-negative floats order like their bit patterns
@@ -958,7 +915,7 @@ Layout proj_layout(const ProjSizes& z)
     L.add(4 * (size_t)z.P);               // 12 transposed decoder params
     L.add(4 * (size_t)z.S * z.C);         // 13 dfeat ([S][C]; matrix-core backward: [C/4][S][4])
     L.add(16 * (size_t)z.S);              // 14 grid coordinates (ix, iy, iz, -) per sample, matrix-core backward
-    L.add(256);                           // 15 max |dL/dfeature| (float bits), matrix-core backward
+    L.add(4 * (size_t)(z.N + 1));         // 15 max |dL/dfeature| (float bits) of the batch, then of each slot
     L.add(4 * (kNodeWGMax + 1));          // 16 first chunk of each node-resident workgroup
     L.add(64 * (size_t)z.K);              // 17 node-major segment records (ot, dt, i, h)
     L.add(z.prop ? 32 * (size_t)z.K : 16);  // 18 node-major proposal cells / CDF rows
@@ -1149,6 +1106,19 @@ int launch_scatter(hipStream_t s, const Args& a, const Plan& p, const float* dfe
 {
     const int32_t gsize = a.g.G * a.g.G * a.g.G * C;
     const size_t smem   = (size_t)gsize * sizeof(float);
+    if (C == 8 && sgc && a.precision == NEAT_PREC_BF16)
+    {
+        // the BF16 form: one pass over all 8 channels in 32-bit fixed point
+        NEAT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_proj_bw2_i32),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        k_proj_bw2_i32<<<(unsigned)std::max<int64_t>(1, a.z.max2), kScatterThreads, smem, s>>>(
+            a.g, p, dfeat, sgc, dmax, grad_grid, grid_partials);
+        NEAT_LAUNCHED();
+        dim3 rg(grid_for(gsize, 256), (unsigned)a.g.N);
+        k_proj_bw2_reduce<<<rg, 256, 0, s>>>(a.g, p, gsize, grid_partials, grad_grid);
+        NEAT_LAUNCHED();
+        return NEAT_OK;
+    }
     NEAT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_proj_bw2<C>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     // the fixed-point form holds half the channels per workgroup as int64 (the same bytes as all of them in FP32)
@@ -1208,7 +1178,7 @@ int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b
     const int Ptot    = a.m.nparams();
     if (stage == 1)
     {
-        k_fill_f32<<<1, 64, 0, s>>>(reinterpret_cast<float*>(b.dmax), 1, 0.f);
+        k_fill_f32<<<grid_for(a.g.N + 1, 256), 256, 0, s>>>(reinterpret_cast<float*>(b.dmax), a.g.N + 1, 0.f);
         NEAT_LAUNCHED();
         if (relu16)
         {

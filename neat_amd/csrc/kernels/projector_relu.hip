@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
         for (int b = 0; b < MT; ++b) dW2[a][b] = f32x16{};
     }
     float b3 = 0.f;
-    float amax = 0.f;
+    float amax = 0.f;  // largest |dL/dfeature| this lane wrote (the batch's goes to dmax[0], each slot's to 1 + slot)
     float* sdz          = s_dz[wave];
     unsigned short* trt = s_tr[wave];
     const uint32_t one2 = opaque_one2();
@@ -183,6 +183,7 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
         node_bounds(g, g.seg_node[p.perm[q0]], nb);
         __syncthreads();
         const int32_t nsamp = cx.beg[nseg];
+        float cmax = 0.f;  // this chunk's
         for (int32_t w0 = 0; w0 < nsamp; w0 += kFwdChunk)
         {
             const int32_t wn = min(kFwdChunk, nsamp - w0);
@@ -233,8 +234,10 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
                         }
                     }
                     // row 0 of the product sits in register 0 of the lane group 0; group 1 takes it from there
+                    // (the shuffle runs on every lane: a lane reading an inactive lane's register gets no data)
                     const float z0 = zacc[0];
-                    const float z  = (hg == 0 ? z0 : __shfl_xor(z0, 32)) + w.b3;
+                    const float zx = __shfl_xor(z0, 32);
+                    const float z  = (hg == 0 ? z0 : zx) + w.b3;
                     dz = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, z, g.beta) : 0.f;
                     bf16x8 dz8;
 #pragma unroll
@@ -323,7 +326,7 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
                 {
                     *reinterpret_cast<float4*>(dfeat + ((int64_t)hg * g.S + base + w0 + i) * 4) =
                         make_float4(dx[0], dx[1], dx[2], dx[3]);
-                    amax = fmaxf(amax, fmaxf(fmaxf(fabsf(dx[0]), fabsf(dx[1])), fmaxf(fabsf(dx[2]), fabsf(dx[3]))));
+                    cmax = fmaxf(cmax, fmaxf(fmaxf(fabsf(dx[0]), fabsf(dx[1])), fmaxf(fabsf(dx[2]), fabsf(dx[3]))));
                 }
             };
             for (int32_t p0 = wave * 64; p0 < wn; p0 += kNodeThreads)
@@ -342,6 +345,10 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
             }
             __syncthreads();
         }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) cmax = fmaxf(cmax, __shfl_xor(cmax, off));
+        if (l == 0) atomicMax(dmax + 1 + slot, __float_as_uint(cmax));
+        amax = fmaxf(amax, cmax);
     }
     // ---- this wave's slice of the partial parameter gradient (packed layout); rows scaled by the output weights
     // where the output weights were folded out of D2 ----

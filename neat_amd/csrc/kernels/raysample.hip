@@ -254,9 +254,9 @@ __global__ void __launch_bounds__(128) k_ray_segments(TreeArgs T, const float* r
 }
 
 // Single-walk sampler: segment i of ray r goes to column entry i * R + r of a scratch of `cap` entries per ray
-// (four columns: node, count, tmin, tmax), with the ray's segment and sample counts; k_ray_compact then copies the
-// scratch into the canonical list once the offsets are known (a ray with more than `cap` segments is walked again
-// there).
+// (four columns: node, count, tmin, tmax), with the ray's segment and sample counts; once the offsets are known,
+// k_ray_rewalk walks the rays with more than `cap` segments again straight into the canonical list and k_ray_compact
+// copies the scratch of the others.
 struct RayScratch
 {
     int32_t* node;
@@ -281,40 +281,89 @@ __global__ void __launch_bounds__(128) k_ray_walk(TreeArgs T, const float* ro, c
     if (max_per_ray) atomicMax(max_per_ray, (int32_t)ns);
 }
 
-__global__ void __launch_bounds__(128) k_ray_compact(TreeArgs T, const float* ro, const float* rd, int64_t R,
-                                                     int32_t max_spn, RayScratch sc, const int64_t* ray_seg_offset,
-                                                     const int64_t* ray_sample_offset, int32_t* seg_node,
-                                                     int32_t* seg_ray, int32_t* seg_count, float* seg_tmin,
-                                                     float* seg_tmax, int64_t* seg_sample_offset)
+// Rays with more segments than the scratch holds (cap): walked again, straight into the canonical list.
+__global__ void __launch_bounds__(128) k_ray_rewalk(TreeArgs T, const float* ro, const float* rd, int64_t R,
+                                                    int32_t max_spn, int64_t cap, const int64_t* ray_seg_offset,
+                                                    int32_t* seg_node, int32_t* seg_count, float* seg_tmin,
+                                                    float* seg_tmax)
 {
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const int64_t base = ray_seg_offset[r];
     const int64_t end  = ray_seg_offset[r + 1];
-    if (end - base > sc.cap)
+    if (end - base <= cap) return;
+    float o[3], d[3];
+    load_ray(ro, rd, r, o, d);
+    int64_t nk, ns;
+    walk_sorted(T, o, d, max_spn, seg_node, seg_count, seg_tmin, seg_tmax, base, 1, end - base, nk, ns);
+}
+
+constexpr int kCompactWaves = 4;
+
+// The scratch copied into the canonical list, one wave per 64 consecutive rays: their segments are the contiguous
+// output range [ray_seg_offset[r0], ray_seg_offset[r0 + 64]), which the wave writes 64 entries at a time. Each lane
+// finds the ray of its entry by a binary search over the 64 ray offsets the lanes hold (largest ray whose first
+// segment is at or before the entry; rays without segments share their successor's offset and are passed over).
+// seg_sample_offset is the exclusive prefix sum of seg_count in canonical order: the ray's sample offset plus a wave
+// scan. Segments of rays over the scratch capacity were written by k_ray_rewalk; only their counts are read back.
+__global__ void __launch_bounds__(64 * kCompactWaves) k_ray_compact(int64_t R, RayScratch sc,
+                                                                   const int64_t* ray_seg_offset,
+                                                                   const int64_t* ray_sample_offset, int32_t* seg_node,
+                                                                   int32_t* seg_ray, int32_t* seg_count,
+                                                                   float* seg_tmin, float* seg_tmax,
+                                                                   int64_t* seg_sample_offset)
+{
+    const int l      = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * kCompactWaves + (threadIdx.x >> 6)) * 64;
+    if (r0 >= R) return;  // uniform over the wave
+    const int64_t nr    = min<int64_t>(64, R - r0);
+    const int64_t off   = ray_seg_offset[r0 + min<int64_t>(l, nr)];  // lanes >= nr hold the range end
+    const int64_t q_beg = __shfl(off, 0);
+    const int64_t q_end = ray_seg_offset[r0 + nr];
+    int64_t run         = ray_sample_offset[r0];
+    for (int64_t qb = q_beg; qb < q_end; qb += 64)
     {
-        float o[3], d[3];
-        load_ray(ro, rd, r, o, d);
-        int64_t nk, ns;
-        walk_sorted(T, o, d, max_spn, seg_node, seg_count, seg_tmin, seg_tmax, base, 1, end - base, nk, ns);
-    }
-    else
-    {
-        for (int64_t i = 0; i < end - base; ++i)
+        const int64_t q = qb + l;
+        int j           = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1)
         {
-            const int64_t e    = i * R + r;
-            seg_node[base + i]  = sc.node[e];
-            seg_count[base + i] = sc.count[e];
-            seg_tmin[base + i]  = sc.tmin[e];
-            seg_tmax[base + i]  = sc.tmax[e];
+            const int cand   = j + step;  // <= 63
+            const int64_t oc = __shfl(off, cand);
+            if (oc <= q) j = cand;
         }
-    }
-    int64_t so = ray_sample_offset[r];
-    for (int64_t q = base; q < end; ++q)
-    {
-        seg_ray[q]           = (int32_t)r;
-        seg_sample_offset[q] = so;
-        so += seg_count[q];
+        const int64_t oj   = __shfl(off, j);
+        const int64_t onx  = __shfl(off, min(j + 1, 63));
+        const int64_t next = j == 63 ? q_end : onx;
+        int32_t cnt        = 0;
+        if (q < q_end)
+        {
+            const int64_t r = r0 + j;
+            if (next - oj > sc.cap)
+            {
+                cnt = seg_count[q];
+            }
+            else
+            {
+                const int64_t e = (q - oj) * R + r;
+                cnt             = sc.count[e];
+                seg_node[q]     = sc.node[e];
+                seg_count[q]    = cnt;
+                seg_tmin[q]     = sc.tmin[e];
+                seg_tmax[q]     = sc.tmax[e];
+            }
+            seg_ray[q] = (int32_t)r;
+        }
+        // inclusive wave scan of the counts
+        int32_t inc = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1)
+        {
+            const int32_t v = __shfl_up(inc, d);
+            if (l >= d) inc += v;
+        }
+        if (q < q_end) seg_sample_offset[q] = run + (inc - cnt);
+        run += __shfl(inc, 63);
     }
 }
 
@@ -484,9 +533,14 @@ extern "C"
         NEAT_REQUIRE(scratch && scratch_bytes >= neat_raysample_scratch_bytes(R, max_depth),
                      "neat_raysample_compact: scratch too small");
         TreeArgs T{pmin, pmax, flags, first_leaf(num_nodes)};
-        k_ray_compact<<<grid_for(R, 128), 128, 0, as_stream(stream)>>>(
-            T, ro, rd, R, max_spn, scratch_view(const_cast<void*>(scratch), R, max_depth), ray_seg_offset,
-            ray_sample_offset, seg_node, seg_ray, seg_count, seg_tmin, seg_tmax, seg_sample_offset);
+        hipStream_t s        = as_stream(stream);
+        const RayScratch sc  = scratch_view(const_cast<void*>(scratch), R, max_depth);
+        k_ray_rewalk<<<grid_for(R, 128), 128, 0, s>>>(T, ro, rd, R, max_spn, sc.cap, ray_seg_offset, seg_node,
+                                                       seg_count, seg_tmin, seg_tmax);
+        NEAT_LAUNCHED();
+        k_ray_compact<<<grid_for(R, 64 * kCompactWaves), 64 * kCompactWaves, 0, s>>>(
+            R, sc, ray_seg_offset, ray_sample_offset, seg_node, seg_ray, seg_count, seg_tmin, seg_tmax,
+            seg_sample_offset);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }

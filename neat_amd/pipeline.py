@@ -81,3 +81,31 @@ def unfused_step(ds, origin, direction, target, jitter=False, seed=0):
     loss, gproj = ops.mse_fp64(proj.reshape(-1), target.reshape(-1))
     ggrid, gparams = unfused_backward(ds, smp, inter, gproj.reshape(1, R))
     return loss, proj, ggrid, gparams, int(smp["ray_sample_offset"][R].item())
+
+
+def train_field(ds, phantom, cfg, steps, batch, precision="bf16", lr_grid=1e-2, lr_decoder=1e-3, seed=0):
+    """Fits the scene's field to the phantom's projections in place: `steps` Adam steps (the reference trainer's
+    learning rates, geometry_ex_ex.cpp:40-49) of the fused projector on `batch` rays per step drawn uniformly over all
+    views and pixels, the targets the phantom's exact line integrals (computed on the device). The BF16 form
+    refreshes its grid copy after every step. Returns (per-step FP64 losses, the BF16 grid copy or None)."""
+    from . import workloads as W
+    dev = ds.grid.device
+    mg, vg = torch.zeros_like(ds.grid), torch.zeros_like(ds.grid)
+    mp, vp = torch.zeros_like(ds.params), torch.zeros_like(ds.params)
+    g16 = ops.grid_to_bf16(ds.grid) if precision == "bf16" else None
+    losses = []
+    for step in range(1, steps + 1):
+        o, d = W.batch_rays(cfg, batch, seed=seed + 7919 * step)
+        to, td = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+        tgt = phantom.line_integrals_torch(to, td).to(torch.float32)
+        segs = sample(ds, to, td, jitter=True, seed=seed + step, emit=False)
+        pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, to, td, segs, ds.grid, ds.params, ds.mlp, True,
+                           seed + step, precision=precision, grid_bf16=g16)
+        loss, gproj = ops.mse_fp64(pj.forward(), tgt)
+        gg, gp = pj.backward(gproj)
+        ops.adam_step(ds.grid, gg, mg, vg, lr_grid, step)
+        ops.adam_step(ds.params, gp, mp, vp, lr_decoder, step)
+        if g16 is not None:
+            ops.grid_to_bf16(ds.grid, out=g16)
+        losses.append(loss)
+    return np.array([float(v) for v in losses]), g16

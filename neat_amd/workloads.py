@@ -58,6 +58,41 @@ class Phantom:
             out += rho * ((q * q).sum(1) <= 1.0)
         return out
 
+    def density_at_torch(self, p):
+        """density_at for a torch tensor p [N,3] on any device (FP64)."""
+        import torch
+        p = p.to(torch.float64)
+        out = torch.zeros(p.shape[0], dtype=torch.float64, device=p.device)
+        for c, a, R, rho in zip(self.center, self.axes, self.rot, self.density):
+            q = (p - torch.as_tensor(c, device=p.device)) @ torch.as_tensor(R.T, device=p.device)
+            q = q / torch.as_tensor(a, device=p.device)
+            out += float(rho) * ((q * q).sum(1) <= 1.0).to(torch.float64)
+        return out
+
+    def line_integrals_torch(self, origin, direction):
+        """line_integrals for torch tensors [R,3] on any device (FP64, the same expressions)."""
+        import torch
+        dev = origin.device
+        o = origin.to(torch.float64)
+        d = direction.to(torch.float64)
+        total = torch.zeros(o.shape[0], dtype=torch.float64, device=dev)
+        dn = torch.linalg.norm(d, dim=1)
+        for c, a, R, rho in zip(self.center, self.axes, self.rot, self.density):
+            Rt = torch.as_tensor(R.T, device=dev)
+            av = torch.as_tensor(a, device=dev)
+            oo = (o - torch.as_tensor(c, device=dev)) @ Rt / av
+            dd = d @ Rt / av
+            A = (dd * dd).sum(1)
+            B = 2 * (oo * dd).sum(1)
+            C = (oo * oo).sum(1) - 1
+            disc = B * B - 4 * A * C
+            ok = disc > 0
+            sq = torch.sqrt(torch.where(ok, disc, torch.zeros_like(disc)))
+            t0 = torch.clamp((-B - sq) / (2 * A), min=0.0)
+            t1 = (-B + sq) / (2 * A)
+            total += float(rho) * torch.where(ok & (t1 > t0), (t1 - t0) * dn, torch.zeros_like(dn))
+        return total
+
     def line_integrals(self, origin, direction):
         """Closed-form integral of the density along each ray (chord length x density), float64 [R]."""
         o = np.asarray(origin, np.float64)
@@ -230,7 +265,7 @@ def batch_rays(cfg, batch, seed=0, det=None):
     return pixel_rays(cfg.views, det, det, v, pix // det, pix % det, det_extent=cfg.det_extent)
 
 
-def phantom_field(scene, cfg, floor=0.01):
+def phantom_field(scene, cfg, floor=0.01, device=None):
     """A grid and decoder whose decoded field is the phantom's density, trilinearly interpolated on every active
     leaf's grid vertices: channel 0 holds softplus_beta^-1(density) (density clamped to >= floor), the other channels
     0, and the decoder passes channel 0 through (relu(x) - relu(-x) in both hidden layers), so that
@@ -242,11 +277,24 @@ def phantom_field(scene, cfg, floor=0.01):
     grid = np.zeros_like(scene["grid"])
     lin = np.arange(G) / (G - 1)
     zz, yy, xx = np.meshgrid(lin, lin, lin, indexing="ij")
-    for node in ids:
-        mn, mx = tree["pos_min"][node].astype(np.float64), tree["pos_max"][node].astype(np.float64)
-        pts = np.stack([mn[0] + xx * (mx[0] - mn[0]), mn[1] + yy * (mx[1] - mn[1]), mn[2] + zz * (mx[2] - mn[2])], -1)
-        rho = np.maximum(scene["phantom"].density_at(pts.reshape(-1, 3)), floor)
-        grid[prefix[node], 0] = (np.log(np.expm1(beta * rho)) / beta).reshape(G, G, G)
+    if device is not None:
+        # all leaves at once on the device (torch FP64)
+        import torch
+        mn = torch.as_tensor(tree["pos_min"][ids].astype(np.float64), device=device)[:, None, :]
+        mx = torch.as_tensor(tree["pos_max"][ids].astype(np.float64), device=device)[:, None, :]
+        unit = torch.as_tensor(np.stack([xx, yy, zz], -1).reshape(1, -1, 3), device=device)
+        pts = (mn + unit * (mx - mn)).reshape(-1, 3)
+        rho = torch.clamp(scene["phantom"].density_at_torch(pts), min=floor)
+        val = (torch.log(torch.expm1(beta * rho)) / beta).reshape(len(ids), G, G, G).cpu().numpy()
+        grid[prefix[ids], 0] = val
+        del pts, rho
+    else:
+        for node in ids:
+            mn, mx = tree["pos_min"][node].astype(np.float64), tree["pos_max"][node].astype(np.float64)
+            pts = np.stack([mn[0] + xx * (mx[0] - mn[0]), mn[1] + yy * (mx[1] - mn[1]),
+                            mn[2] + zz * (mx[2] - mn[2])], -1)
+            rho = np.maximum(scene["phantom"].density_at(pts.reshape(-1, 3)), floor)
+            grid[prefix[node], 0] = (np.log(np.expm1(beta * rho)) / beta).reshape(G, G, G)
     P = []
     W0 = np.zeros((H, C)); W0[0, 0], W0[1, 0] = 1, -1
     W1 = np.zeros((H, H)); W1[0, 0], W1[1, 1] = 1, 1

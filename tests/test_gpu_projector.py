@@ -82,7 +82,10 @@ def test_fused_projector_matches_unfused_chain():
 
 def test_fused_projector_adaptive_tree(oracle):
     """The full-size configuration (depth-5 adaptive tree, 17^3 x 8 grids, decoder 8->64->64->1) on 8192 rays (about
-    0.1 M segments and 1.4 M samples): projection, grid and decoder gradients within 1e-5 of the FP32 oracle."""
+    0.1 M segments and 1.4 M samples): projection and grid gradient within 1e-5 of the FP32 oracle. The decoder
+    gradient sums 1.4 M samples' terms in FP32: the GPU (per-wave partials) and the FP32 oracle (per-thread partials,
+    which depend on its thread count) each sit about 7e-5 from the FP64 oracle, so it is held to the FP64 oracle within
+    1e-4, as in the golden full-size check (all three distances printed)."""
     print("case: fused projector C2 geometry, 8192 rays", flush=True)
     cfg = W.C2
     sc, ds, o, d, to, td = setup(cfg, 8192)
@@ -95,8 +98,13 @@ def test_fused_projector_adaptive_tree(oracle):
     assert rel(proj.cpu().numpy(), ref) <= RTOL
     rgg, rgp = oracle.projector_bwd(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, gproj, jitter=True,
                                     seed=4, prec=0, nthreads=min(8, oracle.max_threads()))
+    _, rgp64 = oracle.projector_bwd(sc["tree"], sc["prefix"], o, d, hseg, sc["grid"], mlp, gproj, jitter=True,
+                                    seed=4, prec=1)
+    gpn = gp.cpu().numpy()
+    print("decoder gradient: gpu vs fp64 %.3e, fp32 oracle vs fp64 %.3e, gpu vs fp32 oracle %.3e"
+          % (rel(gpn, rgp64), rel(rgp, rgp64), rel(gpn, rgp)), flush=True)
     assert rel(gg.cpu().numpy(), rgg) <= RTOL
-    assert rel(gp.cpu().numpy(), rgp) <= RTOL
+    assert rel(gpn, rgp64) <= 1e-4
 
 
 def test_fused_projector_deterministic_forward_and_empty():
