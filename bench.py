@@ -38,9 +38,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--sampling", default="auto", choices=["auto", "proposal", "uniform"],
-                    help="proposal (the default for the fused BF16 path): BF16 + learned proposal form; the field is "
-                         "the phantom fitted on the leaf grids, segments with more than 8 uniform samples get 8 "
-                         "proposal-guided ones unless their slot's observed variation exceeds --proposal-tau, and the "
+                    help="proposal (the default for the fused BF16 path): BF16 + learned proposal form on the field "
+                         "of --field; segments with more than 8 uniform samples get 8 proposal-guided ones where their "
+                         "slot's observed variation is at most --proposal-tau, up to --proposal-mid-samples where it "
+                         "is at most --proposal-tau-uniform, and their uniform samples elsewhere; the "
                          "proposal network takes one online training step every --proposal-train-every steps on 1/32 "
                          "of the batch's rays; the projection error is reported against the FP64 oracle with uniform "
                          "samples. uniform (the default otherwise): the reference's uniform in-node sampling over a "
@@ -54,8 +55,12 @@ def parse():
                          "decoder of the configuration")
     ap.add_argument("--field-steps", type=int, default=300, help="training steps of the trained field (2^18 rays)")
     ap.add_argument("--proposal-tau", type=float, default=0.3,
+                    help="slots whose observed field variation is at most this take 8 proposal samples per segment")
+    ap.add_argument("--proposal-tau-uniform", type=float, default=2.0,
                     help="fallback threshold: segments in slots whose observed field variation exceeds it keep their "
-                         "uniform samples")
+                         "uniform samples (equal to --proposal-tau: two tiers)")
+    ap.add_argument("--proposal-mid-samples", type=int, default=24,
+                    help="proposal samples per segment in slots between the two thresholds")
     ap.add_argument("--proposal-train-every", type=int, default=4,
                     help="the proposal network's online training step runs every this many steps")
     ap.add_argument("--proposal-pretrain", type=int, default=200,
@@ -268,7 +273,8 @@ def main():
     if a.proposal:
         from neat_amd import proposal as PP
         prop = PP.ProposalSampler(dev, num_samples=8, lr=3e-3, seed=a.seed + 7, num_slots=ds.grid.shape[0],
-                                  tau=a.proposal_tau)
+                                  tau=a.proposal_tau, tau_uniform=a.proposal_tau_uniform,
+                                  mid_samples=a.proposal_mid_samples)
         for k in range(a.proposal_pretrain):  # untimed set-up: the network starts trained
             o, d = W.batch_rays(cfg, 4096, seed=a.seed + 5000 + k)
             to, td = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
@@ -327,6 +333,12 @@ def main():
                 "stage": dominant, "kernel_ms": round(stage_ms[dominant], 4)}
 
     err = projection_error(scene, cfg, ds, prop, a, grid_bf16)
+    tiers = None
+    if prop is not None:
+        o0, d0 = batches[0][0], batches[0][1]
+        fr = prop.tier_fractions(pipeline.sample(ds, o0, d0, jitter=False, emit=False), ds.prefix)
+        tiers = {"proposal_8": round(fr[0], 4), f"proposal_{a.proposal_mid_samples}": round(fr[1], 4),
+                 "uniform": round(fr[2], 4), "of": "segments with more than 8 uniform samples (one batch)"}
 
     # comparison forms on the same field and rays, after the timed region: the FP32 forms sample uniformly (the
     # proposal is part of the BF16 form); the first plain-FP32 GPU version (the unfused kernel chain) is the
@@ -376,6 +388,7 @@ def main():
               "samples_per_batch": round(mean_samples),
               "samples_per_s": round(mean_samples / (ms_per_step / 1e3), 1),
               "field": field_info,
+              "proposal_tiers": tiers,
               "projection_error_vs_fp64": err,
               "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
               "build_id": {"libneat_amd": _abi.build_id(), "neat_host": host.load().build_id()}}

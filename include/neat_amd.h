@@ -346,11 +346,15 @@ int neat_proposal_roughness_update(neat_stream_t stream, const float* values, co
                                    const int64_t* node_active_prefix_sum, int32_t num_nodes, int64_t num_slots,
                                    float decay, float* slot_roughness);
 
-/* Uniform in-node sampling as the fallback: segments whose slot's roughness exceeds tau get a zero CDF row and their
- * uniform count back (seg_count, the sampler's counts) in seg_count_out; the others keep what neat_proposal_fwd set. */
+/* Uniform in-node sampling as the fallback, by the slot's roughness r: segments with r <= tau keep what
+ * neat_proposal_fwd set (seg_count_out, seg_cdf); with tau < r <= tau_uniform a segment that neat_proposal_fwd gave
+ * proposal samples gets min(its uniform count, mid_samples) of them (the same CDF row, more cells); with
+ * r > tau_uniform (or outside every slot) a segment gets a zero CDF row and its uniform count back (seg_count, the
+ * sampler's counts). tau_uniform = tau gives the two-tier gate. */
 int neat_proposal_gate(neat_stream_t stream, const int32_t* seg_node, const int32_t* seg_count, int64_t num_segments,
                        const int64_t* node_active_prefix_sum, int32_t num_nodes, int64_t num_slots,
-                       const float* slot_roughness, float tau, float* seg_cdf, int32_t* seg_count_out);
+                       const float* slot_roughness, float tau, float tau_uniform, int32_t mid_samples, float* seg_cdf,
+                       int32_t* seg_count_out);
 
 /* grad_params = gradient of scale * sum_k CE(target_k, softmax(logits_k)) (scale = 1 / num_segments for the mean). */
 size_t neat_proposal_workspace_bytes(int64_t num_segments);

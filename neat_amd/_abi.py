@@ -92,7 +92,7 @@ SIGNATURES.update({
     "neat_proposal_targets": (ctypes.c_int, [_P, _P, _P, _P, _I64, _P]),
     "neat_proposal_workspace_bytes": (_SZ, [_I64]),
     "neat_proposal_roughness_update": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _P, _I32, _I64, _F, _P]),
-    "neat_proposal_gate": (ctypes.c_int, [_P, _P, _P, _I64, _P, _I32, _I64, _P, _F, _P, _P]),
+    "neat_proposal_gate": (ctypes.c_int, [_P, _P, _P, _I64, _P, _I32, _I64, _P, _F, _F, _I32, _P, _P]),
     "neat_proposal_bwd": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I64, _P, _P, _F, _P, _P, _SZ]),
     "neat_adam_step": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double, _I64]),

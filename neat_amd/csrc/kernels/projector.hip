@@ -564,7 +564,9 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
 // gradient stays bit-reproducible. Lanes are sample streams as in
 // k_proj_bw2; a lane keeps the current cell's 8 corners x 8 channels in registers, prescaled, and converts them once
 // per cell (round to nearest).
-__global__ void __launch_bounds__(kScatterThreads) k_proj_bw2_i32(ProjGeom g, Plan p, const float* dfeat,
+constexpr int kScatterThreadsI32 = 512;  // 256 VGPRs per lane: the cell sums and an 8-sample block in registers
+
+__global__ void __launch_bounds__(kScatterThreadsI32) k_proj_bw2_i32(ProjGeom g, Plan p, const float* dfeat,
                                                                   const float4* sgc, const unsigned int* dmax,
                                                                   float* grad_grid, float* partial_grids)
 {
@@ -615,14 +617,17 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2_i32(ProjGeom g, Pl
                 }
             }
         };
-        constexpr int kSB = 2;
-        for (int64_t s0 = s2; s0 < e2; s0 += kSB)
+        // blocks of 8 samples: 128 B of each array per stream and block, a whole cache line of each once the stream
+        // is aligned to it (its first block runs to the next 8-sample boundary), so a line is fetched once
+        constexpr int kSB = 8;
+        for (int64_t s0 = s2; s0 < e2;)
         {
+            const int64_t s1 = min(e2, ((base + s0) / kSB + 1) * kSB - base);
             float4 gb[kSB], da[kSB], db[kSB];
 #pragma unroll
             for (int b = 0; b < kSB; ++b)
             {
-                if (s0 + b < e2)
+                if (s0 + b < s1)
                 {
                     gb[b] = sgc[base + s0 + b];
                     da[b] = d0[s0 + b];
@@ -632,7 +637,7 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2_i32(ProjGeom g, Pl
 #pragma unroll
             for (int b = 0; b < kSB; ++b)
             {
-                if (s0 + b >= e2) break;
+                if (s0 + b >= s1) break;
                 const float4 gc = gb[b];
                 const int x0 = (int)floorf(gc.x), y0 = (int)floorf(gc.y), z0 = (int)floorf(gc.z);
                 const int cc = (x0 + 1) + (G + 2) * ((y0 + 1) + (G + 2) * (z0 + 1));
@@ -661,6 +666,7 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2_i32(ProjGeom g, Pl
                     for (int q = 0; q < 8; ++q) acc[k][q] = fmaf(wk, dv[q], acc[k][q]);
                 }
             }
+            s0 = s1;
         }
         if (cell >= 0) flush();
     }
@@ -1111,7 +1117,7 @@ int launch_scatter(hipStream_t s, const Args& a, const Plan& p, const float* dfe
         // the BF16 form: one pass over all 8 channels in 32-bit fixed point
         NEAT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_proj_bw2_i32),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        k_proj_bw2_i32<<<(unsigned)std::max<int64_t>(1, a.z.max2), kScatterThreads, smem, s>>>(
+        k_proj_bw2_i32<<<(unsigned)std::max<int64_t>(1, a.z.max2), kScatterThreadsI32, smem, s>>>(
             a.g, p, dfeat, sgc, dmax, grad_grid, grid_partials);
         NEAT_LAUNCHED();
         dim3 rg(grid_for(gsize, 256), (unsigned)a.g.N);

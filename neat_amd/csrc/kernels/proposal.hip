@@ -380,20 +380,31 @@ __global__ void __launch_bounds__(256) k_rough_update(const float* values, const
 }
 
 // Fallback to the uniform samples for segments whose slot is rougher than tau.
+// Three tiers by the slot's roughness r: r <= tau keeps what neat_proposal_fwd set; tau < r <= tau_uniform gives a
+// segment that neat_proposal_fwd gave proposal samples min(uniform count, mid_samples) of them (so every long
+// segment of such a slot samples its proposal, none switches back to the uniform rule); r > tau_uniform, or a segment
+// outside every slot, gets its uniform samples back.
 __global__ void __launch_bounds__(256) k_prop_gate(const int32_t* seg_node, const int64_t* prefix, int32_t num_nodes,
-                                                   int64_t num_slots, const float* rough, float tau,
-                                                   const int32_t* count_in, int64_t K, float* cdf, int32_t* count_out)
+                                                   int64_t num_slots, const float* rough, float tau, float tau_uniform,
+                                                   int32_t mid_samples, const int32_t* count_in, int64_t K, float* cdf,
+                                                   int32_t* count_out)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
     const int32_t node = seg_node[k];
     const int64_t slot = (node >= 0 && node < num_nodes) ? prefix[node] : -1;
-    const bool rough_slot = slot < 0 || slot >= num_slots || rough[slot] > tau;
-    if (!rough_slot) return;
-    float4* r = reinterpret_cast<float4*>(cdf + 8 * k);
-    r[0] = make_float4(0.f, 0.f, 0.f, 0.f);
-    r[1] = make_float4(0.f, 0.f, 0.f, 0.f);
-    count_out[k] = count_in[k];
+    const float r      = (slot < 0 || slot >= num_slots) ? INFINITY : rough[slot];
+    if (r <= tau) return;
+    const int32_t n = count_in[k];
+    if (r <= tau_uniform)
+    {
+        if (n > count_out[k]) count_out[k] = min(n, mid_samples);
+        return;
+    }
+    float4* row = reinterpret_cast<float4*>(cdf + 8 * k);
+    row[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+    row[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    count_out[k] = n;
 }
 
 int prop_bwd_workgroups(int64_t K) { return (int)std::max<int64_t>(1, std::min<int64_t>((K + 127) / 128, 256)); }
@@ -490,17 +501,18 @@ extern "C"
 
     int neat_proposal_gate(neat_stream_t stream, const int32_t* seg_node, const int32_t* seg_count,
                            int64_t num_segments, const int64_t* node_active_prefix_sum, int32_t num_nodes,
-                           int64_t num_slots, const float* slot_roughness, float tau, float* seg_cdf,
-                           int32_t* seg_count_out)
+                           int64_t num_slots, const float* slot_roughness, float tau, float tau_uniform,
+                           int32_t mid_samples, float* seg_cdf, int32_t* seg_count_out)
     {
         NEAT_REQUIRE(num_segments >= 0, "neat_proposal_gate: bad segment count");
+        NEAT_REQUIRE(mid_samples >= 1, "neat_proposal_gate: mid_samples %d must be >= 1", mid_samples);
         if (num_segments == 0) return NEAT_OK;
         NEAT_REQUIRE(seg_node && seg_count && node_active_prefix_sum && slot_roughness && seg_cdf && seg_count_out,
                      "neat_proposal_gate: null pointer");
         NEAT_REQUIRE(reinterpret_cast<uintptr_t>(seg_cdf) % 16 == 0, "neat_proposal_gate: seg_cdf must be 16-byte aligned");
         k_prop_gate<<<grid_for(num_segments, 256), 256, 0, as_stream(stream)>>>(
-            seg_node, node_active_prefix_sum, num_nodes, num_slots, slot_roughness, tau, seg_count, num_segments,
-            seg_cdf, seg_count_out);
+            seg_node, node_active_prefix_sum, num_nodes, num_slots, slot_roughness, tau, tau_uniform, mid_samples,
+            seg_count, num_segments, seg_cdf, seg_count_out);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }

@@ -332,10 +332,13 @@ def proposal_roughness_update(values, seg_value_offset, segs, prefix, num_slots,
          float(decay), ptr(roughness))
 
 
-def proposal_gate(segs, seg_count_out, seg_cdf, prefix, num_slots, roughness, tau, stream=None):
+def proposal_gate(segs, seg_count_out, seg_cdf, prefix, num_slots, roughness, tau, tau_uniform=None, mid_samples=8,
+                  stream=None):
+    """neat_proposal_gate: tau_uniform None is tau (two tiers: proposal or uniform)."""
     K = segs["seg_count"].shape[0]
     call("neat_proposal_gate", stream_ptr(stream), ptr(segs["seg_node"]), ptr(segs["seg_count"]), K, ptr(prefix),
-         prefix.shape[0], int(num_slots), ptr(roughness), float(tau), ptr(seg_cdf), ptr(seg_count_out))
+         prefix.shape[0], int(num_slots), ptr(roughness), float(tau), float(tau if tau_uniform is None else tau_uniform),
+         int(mid_samples), ptr(seg_cdf), ptr(seg_count_out))
 
 
 # ---- volume regulariser (GeometryExEx::VolumeRegularizer, geometry_ex_ex.cpp:86-160) ---------------------------------

@@ -28,10 +28,12 @@ def init_params(seed=0):
 
 class ProposalSampler:
     """The network, its Adam state, and the fallback's per-slot roughness table (the largest variation of the field
-    along a segment seen in the slot, decayed every training step): segments in slots rougher than `tau` keep their
-    uniform samples (tau None: no fallback)."""
+    along a segment seen in the slot, decayed every training step): segments in slots rougher than `tau` get
+    `mid_samples` proposal samples if their slot is at most `tau_uniform` rough, and keep their uniform samples
+    otherwise (tau None: no fallback; tau_uniform None: tau, i.e. proposal or uniform)."""
 
-    def __init__(self, device="cuda", num_samples=8, lr=1e-3, seed=0, num_slots=0, tau=None, decay=0.98):
+    def __init__(self, device="cuda", num_samples=8, lr=1e-3, seed=0, num_slots=0, tau=None, decay=0.98,
+                 tau_uniform=None, mid_samples=32):
         self.params = torch.from_numpy(init_params(seed)).to(device)
         self.exp_avg = torch.zeros_like(self.params)
         self.exp_avg_sq = torch.zeros_like(self.params)
@@ -39,6 +41,8 @@ class ProposalSampler:
         self.lr = lr
         self.steps = 0
         self.tau = tau
+        self.tau_uniform = tau_uniform
+        self.mid_samples = mid_samples
         self.decay = decay
         self.roughness = torch.zeros(max(num_slots, 1), dtype=torch.float32, device=device)
         self.num_slots = num_slots
@@ -48,12 +52,28 @@ class ProposalSampler:
         fallback threshold, `prefix` (node -> slot) is needed."""
         cdf, cnt, _ = ops.proposal_fwd(origin, direction, segs, self.params, self.num_samples)
         if self.tau is not None:
-            ops.proposal_gate(segs, cnt, cdf, prefix, self.num_slots, self.roughness, self.tau)
+            ops.proposal_gate(segs, cnt, cdf, prefix, self.num_slots, self.roughness, self.tau, self.tau_uniform,
+                              self.mid_samples)
         out = dict(segs)
         out["seg_count"] = cnt
         out["seg_cdf"] = cdf
         out["num_samples"] = int(cnt.sum().item()) if cnt.numel() else 0
         return out
+
+    def tier_fractions(self, segs, prefix):
+        """Shares of the segments with more than num_samples uniform samples that the gate sends to each tier:
+        (num_samples proposal samples, up to mid_samples proposal samples, uniform)."""
+        n = segs["seg_count"]
+        big = n > self.num_samples
+        nb = max(int(big.sum().item()), 1)
+        slot = prefix[segs["seg_node"].long()]
+        ok = (slot >= 0) & (slot < self.roughness.shape[0])
+        r = torch.where(ok, self.roughness[slot.clamp(0, self.roughness.shape[0] - 1)],
+                        torch.full_like(n, float("inf"), dtype=torch.float32))
+        tu = self.tau if self.tau_uniform is None else self.tau_uniform
+        lo = big & (r <= self.tau)
+        mid = big & (r > self.tau) & (r <= tu)
+        return (float(lo.sum().item()) / nb, float(mid.sum().item()) / nb, 1.0 - float((lo | mid).sum().item()) / nb)
 
     def targets(self, ds, origin, direction, segs_uniform, precision="bf16", grid_bf16=None):
         """Bin targets of a uniformly sampled (a = 0.5) segment list from the field's values."""

@@ -77,11 +77,15 @@ class Phantom:
         d = direction.to(torch.float64)
         total = torch.zeros(o.shape[0], dtype=torch.float64, device=dev)
         dn = torch.linalg.norm(d, dim=1)
+        def rot(x, R):
+            # x @ R.T as three broadcast products (a 3x3 product is too small for a matrix kernel)
+            return (x[:, 0:1] * torch.as_tensor(R[:, 0], device=dev) + x[:, 1:2] * torch.as_tensor(R[:, 1], device=dev)
+                    + x[:, 2:3] * torch.as_tensor(R[:, 2], device=dev))
+
         for c, a, R, rho in zip(self.center, self.axes, self.rot, self.density):
-            Rt = torch.as_tensor(R.T, device=dev)
             av = torch.as_tensor(a, device=dev)
-            oo = (o - torch.as_tensor(c, device=dev)) @ Rt / av
-            dd = d @ Rt / av
+            oo = rot(o - torch.as_tensor(c, device=dev), R) / av
+            dd = rot(d, R) / av
             A = (dd * dd).sum(1)
             B = 2 * (oo * dd).sum(1)
             C = (oo * oo).sum(1) - 1

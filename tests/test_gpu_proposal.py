@@ -121,7 +121,7 @@ def test_trained_proposal_beats_uniform_at_equal_budget():
     print("case: trained proposal vs uniform at 8 samples per segment, phantom field", flush=True)
     cfg = W.C2
     sc = W.make_scene(cfg)
-    sc["grid"], sc["params"] = W.phantom_field(sc, cfg)
+    sc["grid"], sc["params"] = W.phantom_field(sc, cfg, device=DEV)
     ds = pipeline.to_device(sc, cfg)
     ps = proposal.ProposalSampler(DEV, num_samples=8, lr=3e-3, seed=7)
     losses = []
@@ -177,16 +177,42 @@ def test_roughness_table_and_fallback_gate():
     assert np.array_equal(cnt[rough_seg], h["seg_count"][rough_seg]) and np.all(cdf[rough_seg] == 0)
     smooth = ~rough_seg & (h["seg_count"] > 8)
     assert smooth.any() and np.all(cnt[smooth] == 8) and np.all(cdf[smooth, 7] == 1)
+    # three tiers: r <= tau_lo -> 8 proposal samples, tau_lo < r <= tau_u -> min(uniform count, 16) proposal samples
+    # for segments over 8, r > tau_u -> uniform
+    r = ref[slot]
+    n = h["seg_count"]
+    vals = np.unique(r[n > 16])  # thresholds a third and two thirds into the roughness of long segments
+    tau_lo, tau_u = float(vals[len(vals) // 3]), float(vals[2 * len(vals) // 3])
+    ps3 = proposal.ProposalSampler(DEV, num_samples=8, seed=3, num_slots=N, tau=tau_lo, tau_uniform=tau_u,
+                                   mid_samples=16)
+    ps3.roughness.copy_(rough)
+    p3 = ps3.apply(to, td, segs, ds.prefix)
+    cnt3, cdf3 = p3["seg_count"].cpu().numpy(), p3["seg_cdf"].cpu().numpy()
+    lo, mid, hi = r <= tau_lo, (r > tau_lo) & (r <= tau_u), r > tau_u
+    assert lo.any() and mid.any() and hi.any()
+    assert np.array_equal(cnt3[lo], np.minimum(n[lo], 8))
+    assert np.all(cdf3[lo & (n > 8), 7] == 1) and np.all(cdf3[lo & (n <= 8)] == 0)
+    mp = mid & (n > 8)
+    assert (mid & (n > 16)).any()
+    assert np.array_equal(cnt3[mp], np.minimum(n[mp], 16)) and np.all(cdf3[mp, 7] == 1)
+    assert np.array_equal(cnt3[mid & ~mp], n[mid & ~mp]) and np.all(cdf3[mid & ~mp] == 0)
+    assert np.array_equal(cnt3[hi], n[hi]) and np.all(cdf3[hi] == 0)
 
 
-def test_proposal_with_fallback_meets_the_error_bar():
-    print("case: trained proposal + roughness fallback vs the FP64 uniform judge, phantom field", flush=True)
+@pytest.mark.parametrize("tau_uniform,mid", [(None, 8), (2.0, 32)])
+def test_proposal_with_fallback_meets_the_error_bar(tau_uniform, mid):
+    """Two tiers (8 proposal samples or uniform, tau 0.3) and three tiers (up to 32 proposal samples in slots
+    rougher than 0.3 and at most 2.0 rough): each within 1e-3 of the FP64 uniform judge, with fewer samples; the
+    share of segments that fall back to uniform sampling is printed."""
+    print("case: trained proposal + roughness fallback vs the FP64 uniform judge, phantom field", tau_uniform, mid,
+          flush=True)
     from oracle import oracle as O
     cfg = W.C2
     sc = W.make_scene(cfg)
-    sc["grid"], sc["params"] = W.phantom_field(sc, cfg)
+    sc["grid"], sc["params"] = W.phantom_field(sc, cfg, device=DEV)
     ds = pipeline.to_device(sc, cfg)
-    ps = proposal.ProposalSampler(DEV, num_samples=8, lr=3e-3, seed=7, num_slots=ds.grid.shape[0], tau=0.3)
+    ps = proposal.ProposalSampler(DEV, num_samples=8, lr=3e-3, seed=7, num_slots=ds.grid.shape[0], tau=0.3,
+                                  tau_uniform=tau_uniform, mid_samples=mid)
     for step in range(200):
         o, d = W.batch_rays(cfg, 4096, seed=2000 + step)
         to, td = torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV)
@@ -201,7 +227,9 @@ def test_proposal_with_fallback_meets_the_error_bar():
     pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, to, td, psegs, ds.grid, ds.params, ds.mlp, False, 0)
     e = rel(pj.forward().cpu().numpy(), judge)
     s_full, s_prop = int(segs["ray_sample_offset"][-1].item()), psegs["num_samples"]
-    print("proposal + fallback (tau 0.3): error vs FP64 uniform judge %.3e, samples %d of %d (%.2fx fewer)" % (
-        e, s_prop, s_full, s_full / s_prop), flush=True)
+    fr = ps.tier_fractions(segs, ds.prefix)
+    print("proposal + fallback (tau 0.3, tau_uniform %s, mid %d): error vs FP64 uniform judge %.3e, samples %d of %d "
+          "(%.2fx fewer); tiers (8, mid, uniform) of segments over 8 samples: %.3f %.3f %.3f" % (
+              tau_uniform, mid, e, s_prop, s_full, s_full / s_prop, *fr), flush=True)
     assert e <= 1e-3
     assert s_prop < s_full
