@@ -690,18 +690,40 @@ __global__ void __launch_bounds__(kScatterThreadsI32) k_proj_bw2_i32(ProjGeom g,
     }
 }
 
-// Slots cut into several scatter chunks: sum the chunk grids in chunk order. Slots without samples: zero.
+// Slots cut into several scatter chunks: sum the chunk grids in chunk order; slots without samples: zero. The plan
+// lists exactly these slots (redo_slots, in any order: each output element is one thread's sum in chunk order, so the
+// result does not depend on it), and a fixed number of workgroups walks the list's (slot, 1024-element tile) items.
+constexpr int kReduceWG = 1024;
 __global__ void __launch_bounds__(256) k_proj_bw2_reduce(ProjGeom g, Plan p, int32_t gsize, const float* partial_grids,
                                                          float* grad_grid)
 {
-    const int64_t slot = blockIdx.y;
-    const int32_t i    = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= gsize) return;
-    const int64_t c0 = p.chunk_off2[slot], c1 = p.chunk_off2[slot + 1];
-    if (c1 - c0 == 1) return;
-    float acc = 0.f;
-    for (int64_t c = c0; c < c1; ++c) acc += partial_grids[c * gsize + i];
-    grad_grid[slot * gsize + i] = acc;
+    const int32_t tiles = (gsize + 1023) / 1024;
+    const int64_t items = (int64_t)*p.nredo * tiles;
+    for (int64_t item = blockIdx.x; item < items; item += gridDim.x)
+    {
+        const int64_t slot = p.redo_slots[item / tiles];
+        const int32_t i    = (int32_t)(item % tiles) * 1024 + 4 * (int32_t)threadIdx.x;
+        if (i >= gsize) continue;
+        const int64_t c0 = p.chunk_off2[slot], c1 = p.chunk_off2[slot + 1];
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int64_t c = c0; c < c1; ++c)
+        {
+            const float4 v = *reinterpret_cast<const float4*>(partial_grids + c * gsize + i);
+            acc.x += v.x;
+            acc.y += v.y;
+            acc.z += v.z;
+            acc.w += v.w;
+        }
+        *reinterpret_cast<float4*>(grad_grid + slot * gsize + i) = acc;
+    }
+}
+
+// The slots the reduction visits: those with no scatter chunk or with several.
+__global__ void __launch_bounds__(256) k_plan_redo(ProjGeom g, Plan p)
+{
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= g.N) return;
+    if (p.chunk_off2[slot + 1] - p.chunk_off2[slot] != 1) p.redo_slots[atomicAdd(p.nredo, 1)] = (int32_t)slot;
 }
 
 __global__ void __launch_bounds__(256) k_fill_f32(float* p, int64_t n, float v)
@@ -926,6 +948,7 @@ Layout proj_layout(const ProjSizes& z)
     L.add(64 * (size_t)z.K);              // 17 node-major segment records (ot, dt, i, h)
     L.add(z.prop ? 32 * (size_t)z.K : 16);  // 18 node-major proposal cells / CDF rows
     L.add(24 * (size_t)z.K);              // 19 per-segment (dL/do, dL/dd) of the positional gradient
+    L.add(4 * (size_t)(z.N + 1));         // 20 slots the scatter's reduction visits, and their count
     return L;
 }
 
@@ -987,6 +1010,8 @@ Plan plan_view(void* ws, const ProjSizes& z)
     p.rec_i       = reinterpret_cast<int4*>(rec + 32 * z.K);
     p.rec_h       = reinterpret_cast<uint4*>(rec + 48 * z.K);
     p.rec_c       = z.prop ? reinterpret_cast<float4*>(b + L.off[18]) : nullptr;
+    p.redo_slots  = reinterpret_cast<int32_t*>(b + L.off[20]);
+    p.nredo       = p.redo_slots + z.N;
     return p;
 }
 
@@ -1120,8 +1145,7 @@ int launch_scatter(hipStream_t s, const Args& a, const Plan& p, const float* dfe
         k_proj_bw2_i32<<<(unsigned)std::max<int64_t>(1, a.z.max2), kScatterThreadsI32, smem, s>>>(
             a.g, p, dfeat, sgc, dmax, grad_grid, grid_partials);
         NEAT_LAUNCHED();
-        dim3 rg(grid_for(gsize, 256), (unsigned)a.g.N);
-        k_proj_bw2_reduce<<<rg, 256, 0, s>>>(a.g, p, gsize, grid_partials, grad_grid);
+        k_proj_bw2_reduce<<<kReduceWG, 256, 0, s>>>(a.g, p, gsize, grid_partials, grad_grid);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }
@@ -1131,8 +1155,7 @@ int launch_scatter(hipStream_t s, const Args& a, const Plan& p, const float* dfe
     const dim3 grid((unsigned)std::max<int64_t>(1, a.z.max2), sgc ? C / 4 : 1);
     k_proj_bw2<C><<<grid, kScatterThreads, smem, s>>>(a.g, p, dfeat, sgc, dmax, grad_grid, grid_partials);
     NEAT_LAUNCHED();
-    dim3 rg(grid_for(gsize, 256), (unsigned)a.g.N);
-    k_proj_bw2_reduce<<<rg, 256, 0, s>>>(a.g, p, gsize, grid_partials, grad_grid);
+    k_proj_bw2_reduce<<<kReduceWG, 256, 0, s>>>(a.g, p, gsize, grid_partials, grad_grid);
     NEAT_LAUNCHED();
     return NEAT_OK;
 }
@@ -1477,6 +1500,10 @@ extern "C"
         k_plan_chunks<<<grid_for(z.max1 + z.max2, 256), 256, 0, s>>>(a.g, p);
         NEAT_LAUNCHED();
         k_plan_wg<<<grid_for(p.nwg + 1, 256), 256, 0, s>>>(a.g, p);
+        NEAT_LAUNCHED();
+        k_fill_f32<<<1, 64, 0, s>>>(reinterpret_cast<float*>(p.nredo), 1, 0.f);  // zero bits
+        NEAT_LAUNCHED();
+        k_plan_redo<<<grid_for(z.N, 256), 256, 0, s>>>(a.g, p);
         NEAT_LAUNCHED();
         return NEAT_OK;
     }

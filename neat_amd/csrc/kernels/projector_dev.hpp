@@ -75,6 +75,8 @@ struct Plan
     int4* rec_i;     // [K] (canonical index, ray, rank in the ray, sample count)
     uint4* rec_h;    // [K] (ray jitter hash low, high, kind, step bits); kind 0 uniform, 1 proposal cells, 2 CDF
     float4* rec_c;   // [2K] kind 1: cell boundaries 1..7 (and 0); kind 2: the CDF row; null without proposal rows
+    int32_t* redo_slots;  // [N] slots with no or several scatter chunks (the scatter's reduction), *nredo of them
+    int32_t* nredo;
 };
 
 // ---- per-sample geometry (bit-identical to the unfused sampler + ComputeLocalSamples chain) -----------------------

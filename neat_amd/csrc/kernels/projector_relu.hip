@@ -68,6 +68,24 @@ __device__ __forceinline__ bf16x8 keep_active(const bf16x8& f, const bf16x8& h, 
 // relu for the FP32 output-layer dot product (one instruction; -0 may pass through, which adds nothing)
 __device__ __forceinline__ float relu_f32(float z) { return __builtin_amdgcn_fmed3f(z, 0.f, 3.402823466e38f); }
 
+// The softplus output and its derivative (the reference's threshold 20 on beta z, DecodeFeatures,
+// geometry.cpp:123-143) on the hardware's exp2 / log2 / reciprocal: the BF16 form is held to its error bar against
+// the FP64 oracle, not to the FP32 oracle's rounding, and the accurate log1p costs about 70 instructions per sample.
+// log(1 + e) loses e's low bits once e < 2^-24 (beta z below about -17), where the value is below 4e-8 / beta.
+__device__ __forceinline__ float softplus_fast(float z, float beta, float inv_beta)
+{
+    const float t = z * beta;
+    return t > 20.f ? z : __logf(1.f + __expf(t)) * inv_beta;
+}
+
+__device__ __forceinline__ float softplus_d_fast(float z, float beta)
+{
+    const float t = z * beta;
+    if (t > 20.f) return 1.f;
+    const float e = __expf(t);
+    return e * __builtin_amdgcn_rcpf(e + 1.f);
+}
+
 // ---- fragment transposes through LDS ------------------------------------------------------------------------------
 // An "h" fragment (samples on the lanes: lane (n, g) holds rows 16st + 8(j/4) + 4g + j%4 of sample n for slot j) and
 // the "s" fragment of the same 32 x 32 tile (lane (m, g) holds samples crow(8st + j, g) of row m) are transposes of
@@ -238,7 +256,7 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
                     const float z0 = zacc[0];
                     const float zx = __shfl_xor(z0, 32);
                     const float z  = (hg == 0 ? z0 : zx) + w.b3;
-                    dz = valid ? gy * act_d(NEAT_ACT_SOFTPLUS, z, g.beta) : 0.f;
+                    dz = valid ? gy * softplus_d_fast(z, g.beta) : 0.f;
                     bf16x8 dz8;
 #pragma unroll
                     for (int j = 0; j < 8; ++j) dz8[j] = (__bf16)dz;
@@ -396,6 +414,7 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_fwd_relu(ProjGeom g, P
     __shared__ uint4 s_grid[kStagedVerts];
     load_weights_relu<H>(params, w);
     const int l = threadIdx.x & 63, wave = threadIdx.x >> 6, n = l & 31, hg = l >> 5;
+    const float inv_beta = 1.f / g.beta;
     bf16x8 ones = {};
     if (hg == 0)
     {
@@ -469,7 +488,7 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_fwd_relu(ProjGeom g, P
                 const float z = part + __shfl_xor(part, 32) + w.b3;
                 if (i < wn && hg == 0)
                 {
-                    const float y = act_f(NEAT_ACT_SOFTPLUS, z, g.beta);
+                    const float y = softplus_fast(z, g.beta, inv_beta);
                     s_contrib[i]  = y * wgt;
                     if (sample_out) sample_out[base + w0 + i] = y;
                 }

@@ -179,13 +179,13 @@ def test_roughness_table_and_fallback_gate():
     assert smooth.any() and np.all(cnt[smooth] == 8) and np.all(cdf[smooth, 7] == 1)
     # three tiers: r <= tau_lo -> 8 proposal samples, tau_lo < r <= tau_u -> min(uniform count, 16) proposal samples
     # for segments over 8, r > tau_u -> uniform
-    r = ref[slot]
+    # (a table with slot % 3 as every slot's roughness puts a third of the slots in each tier)
+    r = (slot % 3).astype(np.float64)
     n = h["seg_count"]
-    vals = np.unique(r[n > 16])  # thresholds a third and two thirds into the roughness of long segments
-    tau_lo, tau_u = float(vals[len(vals) // 3]), float(vals[2 * len(vals) // 3])
+    tau_lo, tau_u = 0.5, 1.5
     ps3 = proposal.ProposalSampler(DEV, num_samples=8, seed=3, num_slots=N, tau=tau_lo, tau_uniform=tau_u,
                                    mid_samples=16)
-    ps3.roughness.copy_(rough)
+    ps3.roughness.copy_(torch.arange(N, device=DEV).remainder(3).to(torch.float32))
     p3 = ps3.apply(to, td, segs, ds.prefix)
     cnt3, cdf3 = p3["seg_count"].cpu().numpy(), p3["seg_cdf"].cpu().numpy()
     lo, mid, hi = r <= tau_lo, (r > tau_lo) & (r <= tau_u), r > tau_u
