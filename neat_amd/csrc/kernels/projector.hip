@@ -1346,6 +1346,36 @@ __global__ void __launch_bounds__(256) k_proj_ray_grad(ProjGeom g, Plan p, const
             const float x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
             float gix = 0.f, giy = 0.f, giz = 0.f;
             const int64_t smp = s0 + j;
+            // the 8 corner vertices, all channels of each in one (BF16) or C / 4 (FP32) vector loads
+            static_assert(!kBF16 || C == 8, "the BF16 grid has 8 channels");
+            float vv[8][C];
+#pragma unroll
+            for (int c8 = 0; c8 < 8; ++c8)
+            {
+                if constexpr (kBF16)
+                {
+                    const uint4 v = *reinterpret_cast<const uint4*>(gh + cr.off[c8]);
+                    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int h = 0; h < 4; ++h)
+                    {
+                        vv[c8][2 * h]     = __uint_as_float(w[h] << 16);
+                        vv[c8][2 * h + 1] = __uint_as_float(w[h] & 0xFFFF0000u);
+                    }
+                }
+                else
+                {
+#pragma unroll
+                    for (int h = 0; h < C / 4; ++h)
+                    {
+                        const float4 v = *reinterpret_cast<const float4*>(gf + cr.off[c8] + 4 * h);
+                        vv[c8][4 * h]     = v.x;
+                        vv[c8][4 * h + 1] = v.y;
+                        vv[c8][4 * h + 2] = v.z;
+                        vv[c8][4 * h + 3] = v.w;
+                    }
+                }
+            }
 #pragma unroll
             for (int ch = 0; ch < C; ++ch)
             {
@@ -1353,8 +1383,7 @@ __global__ void __launch_bounds__(256) k_proj_ray_grad(ProjGeom g, Plan p, const
                                               : dfeat[smp * C + ch];
                 float val[8];
 #pragma unroll
-                for (int c8 = 0; c8 < 8; ++c8)
-                    val[c8] = kBF16 ? bf16_to_f32(gh[cr.off[c8] + ch]) : gf[cr.off[c8] + ch];
+                for (int c8 = 0; c8 < 8; ++c8) val[c8] = vv[c8][ch];
                 const float m1 = -1.f;
                 gix += m1 * val[0] * (y1 - iy) * (z1 - iz) * gout;
                 gix += val[1] * (y1 - iy) * (z1 - iz) * gout;
