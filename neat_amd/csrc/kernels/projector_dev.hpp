@@ -580,15 +580,12 @@ __device__ __forceinline__ void fetch_pair(const ProjGeom& g, const ChunkCtx& cx
 // Copies a node's BF16 grid (nvert vertices x 8 channels, 16 B per vertex) into LDS, four loads in flight per thread.
 __device__ __forceinline__ void stage_node_grid(const uint4* __restrict__ src, uint4* dst, int32_t nvert)
 {
+    // every load runs (clamped to the last vertex) so that the four stay in registers and are in flight together
     for (int32_t t0 = threadIdx.x; t0 < nvert; t0 += 4 * (int32_t)blockDim.x)
     {
         uint4 v[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-        {
-            const int32_t t = t0 + k * (int32_t)blockDim.x;
-            if (t < nvert) v[k] = src[t];
-        }
+        for (int k = 0; k < 4; ++k) v[k] = src[min(t0 + k * (int32_t)blockDim.x, nvert - 1)];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
         {
