@@ -555,15 +555,16 @@ __global__ void __launch_bounds__(kScatterThreads) k_proj_bw2(ProjGeom g, Plan p
         *reinterpret_cast<float4*>(dst + i) = *reinterpret_cast<const float4*>(sg + i);
 }
 
-// The BF16 form's grid-gradient scatter: all 8 channels in one pass, in 32-bit fixed point. A chunk of n samples
+// The BF16 form's grid-gradient scatter: all 8 channels in one pass, in 32-bit fixed point, from the BF16 backward's
+// 24 B per sample (BF16 dL/dfeature [S][8] and the packed cell coordinate, pack_cell_coord). A chunk of n samples
 // adds at most n |dL/dfeature| (trilinear weights are at most 1 and sum to 1 per sample) to any vertex, so with the
-// scale 2^F, F = 30 - ceil(log2 n) - e (|dL/dfeature| < 2^e over the slot's samples, dmax[1 + slot]), every vertex sum
-// and every rounding of its contributions stays inside int32. The quantum is 2^-F, below 2^-30 n max|dL/dfeature|:
-// with scatter chunks of at most 65536 samples, under 2^-14 of the slot's largest value. (F is clamped to +-100, far
-// outside what float gradients reach, so that 2^F stays finite.) Integer adds are exact and order-independent, so the
-// gradient stays bit-reproducible. Lanes are sample streams as in
-// k_proj_bw2; a lane keeps the current cell's 8 corners x 8 channels in registers, prescaled, and converts them once
-// per cell (round to nearest).
+// scale 2^F, F = 30 - ceil(log2 n) - e (|dL/dfeature| <= 2^e over the slot's samples, dmax[1 + slot]: below 2^e in
+// FP32, which the BF16 rounding lifts to 2^e at most), every vertex sum (at most 2^30 plus the flushes' roundings)
+// stays inside int32. The quantum is 2^-F, below 2^-29 n max|dL/dfeature|: with scatter chunks of at most 65536
+// samples, under 2^-13 of the slot's largest value. (F is clamped to +-100, far outside what float gradients reach,
+// so that 2^F stays finite.) Integer adds are exact and order-independent, so the gradient stays bit-reproducible.
+// Lanes are sample streams as in k_proj_bw2; a lane keeps the current cell's 8 corners x 8 channels in registers,
+// prescaled, and converts them once per cell (round to nearest).
 constexpr int kScatterThreadsI32 = 512;  // 256 VGPRs per lane: the cell sums and an 8-sample block in registers
 
 __global__ void __launch_bounds__(kScatterThreadsI32) k_proj_bw2_i32(ProjGeom g, Plan p, const float* dfeat,
@@ -592,8 +593,8 @@ __global__ void __launch_bounds__(kScatterThreadsI32) k_proj_bw2_i32(ProjGeom g,
     const int64_t L2 = (ns + blockDim.x - 1) / blockDim.x;
     const int64_t s2 = (int64_t)threadIdx.x * L2;
     const int64_t e2 = min(ns, s2 + L2);
-    const float4* d0 = reinterpret_cast<const float4*>(dfeat) + base;          // channels 0..3
-    const float4* d1 = reinterpret_cast<const float4*>(dfeat) + g.S + base;    // channels 4..7
+    const uint2* pc = reinterpret_cast<const uint2*>(sgc) + base;  // packed cell coordinates
+    const uint4* dq = reinterpret_cast<const uint4*>(dfeat) + base;  // dL/dfeature, 8 BF16 channels
     if (s2 < e2)
     {
         int cell = -1;
@@ -617,34 +618,36 @@ __global__ void __launch_bounds__(kScatterThreadsI32) k_proj_bw2_i32(ProjGeom g,
                 }
             }
         };
-        // blocks of 8 samples: 128 B of each array per stream and block, a whole cache line of each once the stream
-        // is aligned to it (its first block runs to the next 8-sample boundary), so a line is fetched once
-        constexpr int kSB = 8;
+        // blocks of 8 samples: 64 B of cell coordinates (half a cache line, the other half is the stream's next
+        // block) and 128 B of dL/dfeature per stream and block, once the stream is aligned to them (its first block
+        // runs to the next 8-sample boundary)
+        constexpr int kSB  = 8;
+        constexpr float kQ = 1.f / 65536.f;
         for (int64_t s0 = s2; s0 < e2;)
         {
             const int64_t s1 = min(e2, ((base + s0) / kSB + 1) * kSB - base);
-            float4 gb[kSB], da[kSB], db[kSB];
+            uint2 cb[kSB];
+            uint4 db[kSB];
 #pragma unroll
             for (int b = 0; b < kSB; ++b)
             {
                 if (s0 + b < s1)
                 {
-                    gb[b] = sgc[base + s0 + b];
-                    da[b] = d0[s0 + b];
-                    db[b] = d1[s0 + b];
+                    cb[b] = pc[s0 + b];
+                    db[b] = dq[s0 + b];
                 }
             }
 #pragma unroll
             for (int b = 0; b < kSB; ++b)
             {
                 if (s0 + b >= s1) break;
-                const float4 gc = gb[b];
-                const int x0 = (int)floorf(gc.x), y0 = (int)floorf(gc.y), z0 = (int)floorf(gc.z);
-                const int cc = (x0 + 1) + (G + 2) * ((y0 + 1) + (G + 2) * (z0 + 1));
+                const uint2 u = cb[b];
+                const int cc  = (int)(u.y >> 16);
                 if (cc != cell)
                 {
                     if (cell >= 0) flush();
                     cell = cc;
+                    const int x0 = (cc & 31) - 1, y0 = ((cc >> 5) & 31) - 1, z0 = (cc >> 10) - 1;
                     cx0 = clamp_g(x0, G);
                     cx1 = clamp_g(x0 + 1, G);
                     cy0 = clamp_g(y0, G);
@@ -652,12 +655,14 @@ __global__ void __launch_bounds__(kScatterThreadsI32) k_proj_bw2_i32(ProjGeom g,
                     cz0 = clamp_g(z0, G);
                     cz1 = clamp_g(z0 + 1, G);
                 }
-                const float fx1 = gc.x - (float)x0, fx0 = (float)(x0 + 1) - gc.x;
-                const float fy1 = gc.y - (float)y0, fy0 = (float)(y0 + 1) - gc.y;
-                const float fz1 = gc.z - (float)z0, fz0 = (float)(z0 + 1) - gc.z;
+                const float fx1 = (float)(u.x & 0xFFFFu) * kQ, fx0 = 1.f - fx1;
+                const float fy1 = (float)(u.x >> 16) * kQ, fy0 = 1.f - fy1;
+                const float fz1 = (float)(u.y & 0xFFFFu) * kQ, fz0 = 1.f - fz1;
                 const float wxy[4] = {fx0 * fy0, fx1 * fy0, fx0 * fy1, fx1 * fy1};
-                const float dv[8] = {da[b].x * scale, da[b].y * scale, da[b].z * scale, da[b].w * scale,
-                                     db[b].x * scale, db[b].y * scale, db[b].z * scale, db[b].w * scale};
+                float dv[8];
+                bf16x8_floats(db[b], dv);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) dv[q] *= scale;
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
                 {
@@ -848,8 +853,13 @@ __global__ void __launch_bounds__(256) k_mfma_bwd(ProjGeom g, Plan p, const floa
                 const f32x16 dx = tile_backward_mfma<P, H, ACT>(w, x4, gy, valid, NEAT_ACT_SOFTPLUS, g.beta, sb, acc);
                 if (valid)
                 {
-                    *reinterpret_cast<float4*>(dfeat + ((int64_t)hg * g.S + base + w0 + i) * 4) =
-                        make_float4(dx[0], dx[1], dx[2], dx[3]);
+                    // FP32 form: [2][S][4] (channel halves); BF16 form: [S][8] BF16, this lane's 4 channels
+                    if constexpr (P::kBF16)
+                        *reinterpret_cast<uint2*>(reinterpret_cast<unsigned short*>(dfeat) + (base + w0 + i) * 8 +
+                                                  4 * hg) = bf16x4_bits(dx[0], dx[1], dx[2], dx[3]);
+                    else
+                        *reinterpret_cast<float4*>(dfeat + ((int64_t)hg * g.S + base + w0 + i) * 4) =
+                            make_float4(dx[0], dx[1], dx[2], dx[3]);
                     cmax = fmaxf(cmax, fmaxf(fmaxf(fabsf(dx[0]), fabsf(dx[1])), fmaxf(fabsf(dx[2]), fabsf(dx[3]))));
                 }
             };
@@ -941,8 +951,8 @@ Layout proj_layout(const ProjSizes& z)
     L.add(std::max<size_t>(z.sort_temp, scan_workspace_bytes(z.K + 1)));  // 10 sort / scan temp
     L.add(4 * z.K);                       // 11 seg_partial (forward)
     L.add(4 * (size_t)z.P);               // 12 transposed decoder params
-    L.add(4 * (size_t)z.S * z.C);         // 13 dfeat ([S][C]; matrix-core backward: [C/4][S][4])
-    L.add(16 * (size_t)z.S);              // 14 grid coordinates (ix, iy, iz, -) per sample, matrix-core backward
+    L.add(4 * (size_t)z.S * z.C);         // 13 dfeat ([S][C]; FP32 matrix-core: [C/4][S][4]; BF16: [S][8] BF16)
+    L.add(16 * (size_t)z.S);              // 14 grid coordinates per sample, matrix-core backward (BF16: packed, 8 B)
     L.add(4 * (size_t)(z.N + 1));         // 15 max |dL/dfeature| (float bits) of the batch, then of each slot
     L.add(4 * (kNodeWGMax + 1));          // 16 first chunk of each node-resident workgroup
     L.add(64 * (size_t)z.K);              // 17 node-major segment records (ot, dt, i, h)
@@ -1315,7 +1325,8 @@ __device__ __forceinline__ void record_sample(const ProjGeom& g, const Plan& p, 
 // backward's dL/dfeature (IndirectGridSample3D.cu:752-790, the terms in the reference's order), through the local
 // coordinates' backward (grad * 2) / (max - min) onto the position o + d t, summed over the segment's samples in j
 // order into (dL/do, dL/dd); written at the canonical segment index. Segments without a grid slot get zeros.
-// kHalfMajor: dL/dfeature laid out [C/4][S][4] (the matrix-core backward), else [S][C].
+// dL/dfeature: the BF16 backward's [S][8] BF16 (kBF16); otherwise [C/4][S][4] (kHalfMajor: the FP32 matrix-core
+// backward) or [S][C].
 template <int C, bool kBF16, bool kHalfMajor>
 __global__ void __launch_bounds__(256) k_proj_ray_grad(ProjGeom g, Plan p, const void* grid_lowp, const float* dfeat,
                                                        float* seg_grad)
@@ -1376,11 +1387,19 @@ __global__ void __launch_bounds__(256) k_proj_ray_grad(ProjGeom g, Plan p, const
                     }
                 }
             }
+            float gq[C];
+            if constexpr (kBF16)
+                bf16x8_floats(reinterpret_cast<const uint4*>(dfeat)[smp], gq);
+            else
+            {
+#pragma unroll
+                for (int ch = 0; ch < C; ++ch)
+                    gq[ch] = kHalfMajor ? dfeat[((int64_t)(ch >> 2) * g.S + smp) * 4 + (ch & 3)] : dfeat[smp * C + ch];
+            }
 #pragma unroll
             for (int ch = 0; ch < C; ++ch)
             {
-                const float gout = kHalfMajor ? dfeat[((int64_t)(ch >> 2) * g.S + smp) * 4 + (ch & 3)]
-                                              : dfeat[smp * C + ch];
+                const float gout = gq[ch];
                 float val[8];
 #pragma unroll
                 for (int c8 = 0; c8 < 8; ++c8) val[c8] = vv[c8][ch];

@@ -214,6 +214,41 @@ __device__ __forceinline__ Corners corners_cl(const float uv[3], int G, int C)
     return corners_ix(grid_coord(uv[0], G), grid_coord(uv[1], G), grid_coord(uv[2], G), G, C);
 }
 
+// ---- the BF16 form's per-sample hand-off from the backward to the grid-gradient scatter (24 B per sample) --------
+// dL/dfeature as 8 BF16 channels ([S][8], round to nearest even) and the grid coordinate packed into 8 B: the cell
+// (x0 + 1, y0 + 1, z0 + 1 in 5 bits each, x0 = floor(ix) clamped to [-1, 30]; grids here have G <= 17) and the
+// position inside it (ix - x0, ...) as 16-bit fractions in units of 2^-16 (rounded to nearest, at most 65535):
+// x = fx | fy << 16, y = fz | cell << 16, so y >> 16 alone tells the cells apart. The fractions' quantum, 2^-16 of a
+// cell, is far below the BF16 rounding of dL/dfeature that the same weights multiply.
+__device__ __forceinline__ uint2 pack_cell_coord(float ix, float iy, float iz)
+{
+    auto frac16 = [](float f, float f0) { return (uint32_t)fminf(__builtin_rintf((f - f0) * 65536.f), 65535.f); };
+    auto cell5  = [](float f0) { return (uint32_t)min(31, max(0, (int)f0 + 1)); };
+    const float x0 = floorf(ix), y0 = floorf(iy), z0 = floorf(iz);
+    return make_uint2(frac16(ix, x0) | frac16(iy, y0) << 16,
+                      frac16(iz, z0) | (cell5(x0) | cell5(y0) << 5 | cell5(z0) << 10) << 16);
+}
+
+__device__ __forceinline__ uint2 bf16x4_bits(float a, float b, float c, float d)
+{
+    typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+    typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+    const bf16x4_t v = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
+    const u32x2_t u  = __builtin_bit_cast(u32x2_t, v);
+    return make_uint2(u[0], u[1]);
+}
+
+__device__ __forceinline__ void bf16x8_floats(const uint4& v, float (&x)[8])
+{
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+    {
+        x[2 * h]     = __uint_as_float(w[h] << 16);
+        x[2 * h + 1] = __uint_as_float(w[h] & 0xFFFF0000u);
+    }
+}
+
 template <int C>
 __device__ __forceinline__ void gather_cl(const float* node_grid, const Corners& c, float (&x)[C])
 {
@@ -524,6 +559,8 @@ __device__ __forceinline__ void node_bounds(const ProjGeom& g, int32_t node, Nod
 // sample (lane l: sample p0 + l) and the lane groups swap their corners, so that each lane holds the corners of
 // sample p0 + n (tile A) and p0 + 32 + n (tile B) and gathers its 4 channels of both. v is the sample weight
 // (forward) or dL/dy = dL/dproj x weight (backward); samples past the window have zero weights, features and v.
+// The backward also stores each sample's grid coordinate for the scatter: (ix, iy, iz, 0) in the FP32 form, the
+// packed cell and fractions (pack_cell_coord) in the BF16 form.
 struct PairIn
 {
     float xa[4], xb[4];
@@ -546,7 +583,10 @@ __device__ __forceinline__ void fetch_pair(const ProjGeom& g, const ChunkCtx& cx
         const float ix = grid_coord(smp.uv[0], g.G), iy = grid_coord(smp.uv[1], g.G), iz = grid_coord(smp.uv[2], g.G);
         cr = corners_ix(ix, iy, iz, g.G, 8);
         v  = kBwd ? cx.gproj[t] * smp.w : smp.w;
-        if (kBwd) sgc[base + w0 + ii] = make_float4(ix, iy, iz, 0.f);
+        if constexpr (kBwd && kBF16)
+            reinterpret_cast<uint2*>(sgc)[base + w0 + ii] = pack_cell_coord(ix, iy, iz);
+        else if constexpr (kBwd)
+            sgc[base + w0 + ii] = make_float4(ix, iy, iz, 0.f);
     }
     else
     {

@@ -342,8 +342,9 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
                 }
                 if (valid)
                 {
-                    *reinterpret_cast<float4*>(dfeat + ((int64_t)hg * g.S + base + w0 + i) * 4) =
-                        make_float4(dx[0], dx[1], dx[2], dx[3]);
+                    // dL/dfeature [S][8] in BF16: this lane's channels 4hg..4hg+3
+                    *reinterpret_cast<uint2*>(reinterpret_cast<unsigned short*>(dfeat) + (base + w0 + i) * 8 +
+                                              4 * hg) = bf16x4_bits(dx[0], dx[1], dx[2], dx[3]);
                     cmax = fmaxf(cmax, fmaxf(fmaxf(fabsf(dx[0]), fabsf(dx[1])), fmaxf(fabsf(dx[2]), fabsf(dx[3]))));
                 }
             };
