@@ -233,3 +233,49 @@ def test_proposal_with_fallback_meets_the_error_bar(tau_uniform, mid):
               tau_uniform, mid, e, s_prop, s_full, s_full / s_prop, *fr), flush=True)
     assert e <= 1e-3
     assert s_prop < s_full
+
+
+def test_trained_field_three_tier_gate_at_bench_settings():
+    """The bench's proposal form on a field this project trained (the phantom fitted on the leaf grids, then Adam
+    steps of the fused BF16 projector against the phantom's projections, as pipeline.train_field does for the bench,
+    here 100 steps on 2^16 rays): the three-tier gate at the bench's settings (tau 0.3, tau_uniform 2.0, at most 24
+    proposal samples), the network trained online for 200 steps. Against the FP64 oracle with uniform samples on the
+    same segments: the FP32 projection with proposal samples within 1e-3 (the sampling error alone; the BF16 form's
+    is printed), fewer samples, and fewer than 20% of the long segments sent back to uniform sampling."""
+    print("case: trained field, three-tier gate (0.3, 2.0, 24) vs the FP64 uniform judge", flush=True)
+    from oracle import oracle as O
+    cfg = W.C2
+    sc = W.make_scene(cfg)
+    sc["grid"], sc["params"] = W.phantom_field(sc, cfg, device=DEV)
+    ds = pipeline.to_device(sc, cfg)
+    losses, g16 = pipeline.train_field(ds, sc["phantom"], cfg, 100, 1 << 16, seed=31)
+    print("field training loss: first %.3e, last %.3e" % (losses[0], losses[-1]), flush=True)
+    assert losses[-1] < losses[0]
+    sc["grid"], sc["params"] = ds.grid.cpu().contiguous().numpy(), ds.params.cpu().numpy()
+    ps = proposal.ProposalSampler(DEV, num_samples=8, lr=3e-3, seed=7, num_slots=ds.grid.shape[0], tau=0.3,
+                                  tau_uniform=2.0, mid_samples=24)
+    for step in range(200):
+        o, d = W.batch_rays(cfg, 4096, seed=3000 + step)
+        to, td = torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV)
+        ps.train_step(ds, to, td, pipeline.sample(ds, to, td, jitter=False, emit=False), precision="bf16",
+                      grid_bf16=g16)
+    o, d = W.batch_rays(cfg, 2048, seed=97)
+    to, td = torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV)
+    segs = pipeline.sample(ds, to, td, jitter=False, emit=False)
+    judge = O.projector_fwd(sc["tree"], sc["prefix"], o, d, host_segs(segs), sc["grid"],
+                            dict(sc["mlp"], params=sc["params"]), prec=1)
+    psegs = ps.apply(to, td, segs, ds.prefix)
+
+    def project(precision):
+        pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, to, td, psegs, ds.grid, ds.params, ds.mlp, False, 0,
+                           precision=precision, grid_bf16=g16 if precision == "bf16" else None)
+        return pj.forward().cpu().numpy()
+
+    e32, e16 = rel(project("fp32"), judge), rel(project("bf16"), judge)
+    s_full, s_prop = int(segs["ray_sample_offset"][-1].item()), psegs["num_samples"]
+    fr = ps.tier_fractions(segs, ds.prefix)
+    print("error vs FP64 uniform judge: FP32 %.3e, BF16 %.3e; samples %d of %d (%.2fx fewer); tiers (8, 24, uniform) "
+          "of segments over 8 samples: %.3f %.3f %.3f" % (e32, e16, s_prop, s_full, s_full / s_prop, *fr), flush=True)
+    assert e32 <= 1e-3
+    assert s_prop < s_full
+    assert fr[2] < 0.2
