@@ -290,7 +290,10 @@ int neat_projector_fwd(neat_stream_t stream, const neat_projector_args* args, in
                        void* workspace, size_t workspace_bytes);
 
 /* Gradients of sum_r grad_proj[r] * proj[r]: grad_grid (same layout as the grid, fully written) and grad_params
- * (packed, written). Samples are recomputed; no activations are kept from the forward. */
+ * (packed, written). No activations are kept from the forward; the BF16 form with relu hidden layers reads each
+ * sample's BF16 features, weight and grid cell, which neat_projector_fwd leaves in the workspace (24 B per sample,
+ * the bits it computed), instead of recomputing them: like the reference's autograd backward, it must follow
+ * neat_projector_fwd on the same workspace and arguments. The other forms recompute the samples. */
 int neat_projector_bwd(neat_stream_t stream, const neat_projector_args* args, const float* grad_proj,
                        float* grad_grid, float* grad_params, void* workspace, size_t workspace_bytes);
 

@@ -1022,6 +1022,12 @@ Plan plan_view(void* ws, const ProjSizes& z)
     p.rec_c       = z.prop ? reinterpret_cast<float4*>(b + L.off[18]) : nullptr;
     p.redo_slots  = reinterpret_cast<int32_t*>(b + L.off[20]);
     p.nredo       = p.redo_slots + z.N;
+    // region 13 holds 32 B per sample with 8 channels: the BF16 dL/dfeature in the first half, the feature cache in
+    // the second; region 14 (16 B per sample): the packed cell coordinates, then the weights
+    const bool c8 = z.C == 8;
+    p.feat_cache  = c8 ? reinterpret_cast<uint4*>(b + L.off[13] + 16 * (size_t)z.S) : nullptr;
+    p.w_cache     = c8 ? reinterpret_cast<float*>(b + L.off[14] + 8 * (size_t)z.S) : nullptr;
+    p.cell_cache  = c8 ? reinterpret_cast<uint2*>(b + L.off[14]) : nullptr;
     return p;
 }
 
@@ -1221,9 +1227,9 @@ int launch_bw_mfma(hipStream_t s, const Args& a, const Plan& p, const BwdBufs& b
         NEAT_LAUNCHED();
         if (relu16)
         {
-            NEAT_REQUIRE(launch_mfma_bwd_relu(s, H, a.g, p, a.params, a.grid_bf16, b.grad_proj, b.dfeat, b.sgc, b.dmax,
-                                              b.partials, Ptot) == 0,
-                         "k_mfma_bwd_relu launch failed");
+            // reads the per-sample caches neat_projector_fwd left in this workspace
+            const int rc = launch_mfma_bwd_relu(s, H, a.g, p, a.params, b.grad_proj, b.dfeat, b.dmax, b.partials, Ptot);
+            NEAT_REQUIRE(rc == 0, "k_mfma_bwd_relu launch failed");
             return NEAT_OK;
         }
         const int nwg = mfma_bwd_workgroups(a.z);

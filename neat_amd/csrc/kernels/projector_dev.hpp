@@ -77,6 +77,12 @@ struct Plan
     float4* rec_c;   // [2K] kind 1: cell boundaries 1..7 (and 0); kind 2: the CDF row; null without proposal rows
     int32_t* redo_slots;  // [N] slots with no or several scatter chunks (the scatter's reduction), *nredo of them
     int32_t* nredo;
+    // what the BF16 relu forward leaves for its backward, per sample in node-major order: the features as 8 BF16
+    // channels (the decoder's first-layer input, bit for bit), the sample weight, and the packed cell coordinate
+    // (pack_cell_coord; the scatter's input as well). Null where the workspace has no room (4 channels).
+    uint4* feat_cache;
+    float* w_cache;
+    uint2* cell_cache;
 };
 
 // ---- per-sample geometry (bit-identical to the unfused sampler + ComputeLocalSamples chain) -----------------------
@@ -560,7 +566,8 @@ __device__ __forceinline__ void node_bounds(const ProjGeom& g, int32_t node, Nod
 // sample p0 + n (tile A) and p0 + 32 + n (tile B) and gathers its 4 channels of both. v is the sample weight
 // (forward) or dL/dy = dL/dproj x weight (backward); samples past the window have zero weights, features and v.
 // The backward also stores each sample's grid coordinate for the scatter: (ix, iy, iz, 0) in the FP32 form, the
-// packed cell and fractions (pack_cell_coord) in the BF16 form.
+// packed cell and fractions (pack_cell_coord) in the BF16 form. A BF16 forward given sgc and w_out stores the packed
+// coordinate and the sample weight (what its backward reads instead of recomputing them).
 struct PairIn
 {
     float xa[4], xb[4];
@@ -570,7 +577,7 @@ struct PairIn
 template <bool kBF16, bool kBwd>
 __device__ __forceinline__ void fetch_pair(const ProjGeom& g, const ChunkCtx& cx, int32_t p0, int32_t w0,
                                            int32_t wn, int64_t base, const NodeBox& nb, const void* ngrid,
-                                           float4* sgc, PairIn& in)
+                                           float4* sgc, PairIn& in, float* w_out = nullptr)
 {
     const int l = threadIdx.x & 63, hg = l >> 5;
     const int32_t ii = p0 + l;
@@ -583,10 +590,15 @@ __device__ __forceinline__ void fetch_pair(const ProjGeom& g, const ChunkCtx& cx
         const float ix = grid_coord(smp.uv[0], g.G), iy = grid_coord(smp.uv[1], g.G), iz = grid_coord(smp.uv[2], g.G);
         cr = corners_ix(ix, iy, iz, g.G, 8);
         v  = kBwd ? cx.gproj[t] * smp.w : smp.w;
-        if constexpr (kBwd && kBF16)
-            reinterpret_cast<uint2*>(sgc)[base + w0 + ii] = pack_cell_coord(ix, iy, iz);
+        if constexpr (kBF16)
+        {
+            if (kBwd || sgc) reinterpret_cast<uint2*>(sgc)[base + w0 + ii] = pack_cell_coord(ix, iy, iz);
+            if (!kBwd && w_out) w_out[base + w0 + ii] = smp.w;
+        }
         else if constexpr (kBwd)
+        {
             sgc[base + w0 + ii] = make_float4(ix, iy, iz, 0.f);
+        }
     }
     else
     {
@@ -641,9 +653,9 @@ int launch_mfma_fwd_relu(hipStream_t s, int H, const ProjGeom& g, const Plan& p,
                          const void* grid_bf16, float* seg_partial, float* sample_out);
 
 // Launches k_mfma_bwd_relu<H> (H = 32 or 64) on the plan's p.nwg node-resident workgroups of kNodeThreads; every
-// wave writes one parameter-gradient slice (p.nwg * kNodeWaves slices).
+// wave writes one parameter-gradient slice (p.nwg * kNodeWaves slices). It reads the per-sample caches that
+// k_mfma_fwd_relu wrote into the same plan: the forward must have run on it with the same arguments.
 int launch_mfma_bwd_relu(hipStream_t s, int H, const ProjGeom& g, const Plan& p, const float* params,
-                         const void* grid_bf16, const float* grad_proj, float* dfeat, float4* sgc, unsigned int* dmax,
-                         float* partials, int32_t P_total);
+                         const float* grad_proj, float* dfeat, unsigned int* dmax, float* partials, int32_t P_total);
 
 }  // namespace neat

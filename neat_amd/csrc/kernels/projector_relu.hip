@@ -137,20 +137,21 @@ __device__ __forceinline__ void transpose_tile(unsigned short* tile, const bf16x
     sf[1] = tr_load(tile, 1);
 }
 
-// Node-resident: workgroup b walks forward chunks [wg_first[b], wg_first[b + 1]) and stages the BF16 grid of the
-// current slot in LDS whenever the slot changes (two waves per SIMD, one workgroup per CU).
+// Node-resident like the forward: workgroup b walks forward chunks [wg_first[b], wg_first[b + 1]) (two waves per
+// SIMD, one workgroup per CU). Each sample's features and weight come from what the forward left in the plan's
+// caches (feat_cache, w_cache: the very bits it computed), so the backward stages no node grid and recomputes no
+// sample geometry; the packed cell coordinates the scatter reads are the forward's as well.
 template <int H>
 __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, Plan p, const float* __restrict__ params,
-                                                                   const void* grid_lowp, const float* grad_proj,
-                                                                   float* dfeat, float4* sgc, unsigned int* dmax,
-                                                                   float* partials, int32_t P_total)
+                                                                   const float* grad_proj, float* dfeat,
+                                                                   unsigned int* dmax, float* partials,
+                                                                   int32_t P_total)
 {
     constexpr int MT = H / 32, C = 8;
     typedef PrecBF16 P;
     __shared__ WeightLDSRelu<H> w;
     __shared__ ChunkCtx cx;
     __shared__ __align__(16) float s_dz[kNodeWaves][32];
-    __shared__ uint4 s_grid[kStagedVerts];
     __shared__ __align__(16) unsigned short s_tr[kNodeWaves][32 * kTrRow];
     load_weights_relu<H>(params, w);
     __syncthreads();  // the final partial write reads w.w3 even in a workgroup that gets no chunk
@@ -181,8 +182,7 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
     float* sdz          = s_dz[wave];
     unsigned short* trt = s_tr[wave];
     const uint32_t one2 = opaque_one2();
-    const int32_t nvert = g.G * g.G * g.G;
-    int32_t staged      = -1;
+    const uint2* fc     = reinterpret_cast<const uint2*>(p.feat_cache);  // [S][2]: a lane group's 4 channels each
     const int32_t cend  = p.wg_first[blockIdx.x + 1];
     for (int32_t c = p.wg_first[blockIdx.x]; c < cend; ++c)
     {
@@ -190,15 +190,8 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
         const int32_t q0 = p.chunk1_q0[c], q1 = p.chunk1_q1[c], nseg = q1 - q0;
         if (nseg == 0) continue;
         const int64_t base = p.nm_soff[q0];
-        __syncthreads();  // the previous chunk's readers of cx and s_grid are done
-        if (slot != staged)
-        {
-            stage_node_grid(static_cast<const uint4*>(grid_lowp) + (int64_t)slot * nvert, s_grid, nvert);
-            staged = slot;
-        }
+        __syncthreads();  // the previous chunk's readers of cx are done
         load_chunk_ctx(g, p, q0, q1, base, grad_proj, cx);
-        NodeBox nb;
-        node_bounds(g, g.seg_node[p.perm[q0]], nb);
         __syncthreads();
         const int32_t nsamp = cx.beg[nseg];
         float cmax = 0.f;  // this chunk's
@@ -350,16 +343,24 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
             };
             for (int32_t p0 = wave * 64; p0 < wn; p0 += kNodeThreads)
             {
-                PairIn cur;
-                fetch_pair<true, true>(g, cx, p0, w0, wn, base, nb, s_grid, sgc, cur);
 #pragma unroll 1
                 for (int h = 0; h < 2; ++h)
                 {
                     if (h == 1 && p0 + 32 >= wn) break;
-                    float xs[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) xs[j] = h ? cur.xb[j] : cur.xa[j];
-                    tile(p0 + 32 * h + n, xs, h ? cur.vb : cur.va);
+                    // this lane's 4 channels of sample i and dL/dy = dL/dproj x weight (zeros past the window)
+                    const int32_t i = p0 + 32 * h + n;
+                    float xs[4]     = {0.f, 0.f, 0.f, 0.f};
+                    float gy        = 0.f;
+                    if (i < wn)
+                    {
+                        const uint2 raw = fc[(base + w0 + i) * 2 + hg];
+                        xs[0]           = __uint_as_float(raw.x << 16);
+                        xs[1]           = __uint_as_float(raw.x & 0xFFFF0000u);
+                        xs[2]           = __uint_as_float(raw.y << 16);
+                        xs[3]           = __uint_as_float(raw.y & 0xFFFF0000u);
+                        gy              = cx.gproj[cx.sq[i]] * p.w_cache[base + w0 + i];
+                    }
+                    tile(i, xs, gy);
                 }
             }
             __syncthreads();
@@ -496,8 +497,20 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_fwd_relu(ProjGeom g, P
             };
             for (int32_t p0 = wave * 64; p0 < wn; p0 += kNodeThreads)
             {
+                // the fetch also leaves each sample's packed cell coordinate and weight for the backward
                 PairIn cur;
-                fetch_pair<true, false>(g, cx, p0, w0, wn, base, nb, s_grid, nullptr, cur);
+                fetch_pair<true, false>(g, cx, p0, w0, wn, base, nb, s_grid, reinterpret_cast<float4*>(p.cell_cache),
+                                        cur, p.w_cache);
+                if (p.feat_cache)
+                {
+                    // and this lane's 4 channels of both samples, in BF16 as the first layer takes them
+                    unsigned short* fc = reinterpret_cast<unsigned short*>(p.feat_cache) + (base + w0 + p0 + n) * 8 +
+                                         4 * hg;
+                    if (p0 + n < wn)
+                        *reinterpret_cast<uint2*>(fc) = bf16x4_bits(cur.xa[0], cur.xa[1], cur.xa[2], cur.xa[3]);
+                    if (p0 + 32 + n < wn)
+                        *reinterpret_cast<uint2*>(fc + 32 * 8) = bf16x4_bits(cur.xb[0], cur.xb[1], cur.xb[2], cur.xb[3]);
+                }
                 tile(p0 + n, cur.xa, cur.va);
                 if (p0 + 32 < wn) tile(p0 + 32 + n, cur.xb, cur.vb);
             }
@@ -527,15 +540,13 @@ int launch_mfma_fwd_relu(hipStream_t s, int H, const ProjGeom& g, const Plan& p,
 }
 
 int launch_mfma_bwd_relu(hipStream_t s, int H, const ProjGeom& g, const Plan& p, const float* params,
-                         const void* grid_bf16, const float* grad_proj, float* dfeat, float4* sgc, unsigned int* dmax,
-                         float* partials, int32_t P_total)
+                         const float* grad_proj, float* dfeat, unsigned int* dmax, float* partials, int32_t P_total)
 {
+    if (!p.feat_cache || !p.w_cache) return -1;
     if (H == 64)
-        k_mfma_bwd_relu<64><<<p.nwg, kNodeThreads, 0, s>>>(g, p, params, grid_bf16, grad_proj, dfeat, sgc, dmax,
-                                                            partials, P_total);
+        k_mfma_bwd_relu<64><<<p.nwg, kNodeThreads, 0, s>>>(g, p, params, grad_proj, dfeat, dmax, partials, P_total);
     else
-        k_mfma_bwd_relu<32><<<p.nwg, kNodeThreads, 0, s>>>(g, p, params, grid_bf16, grad_proj, dfeat, sgc, dmax,
-                                                            partials, P_total);
+        k_mfma_bwd_relu<32><<<p.nwg, kNodeThreads, 0, s>>>(g, p, params, grad_proj, dfeat, dmax, partials, P_total);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
