@@ -343,7 +343,8 @@ __global__ void __launch_bounds__(kNodeThreads, 2) k_mfma_bwd_relu(ProjGeom g, P
             };
             for (int32_t p0 = wave * 64; p0 < wn; p0 += kNodeThreads)
             {
-#pragma unroll 1
+                // both tiles' code inline (256 VGPRs, no spills), so that the scheduler can overlap them
+#pragma unroll
                 for (int h = 0; h < 2; ++h)
                 {
                     if (h == 1 && p0 + 32 >= wn) break;
