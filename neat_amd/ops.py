@@ -187,7 +187,9 @@ def grid_to_bf16(grid, stream=None, out=None):
 class Projector:
     """Fused projector over one batch's canonical segment list (neat_projector_plan / _fwd / _bwd).
 
-    Holds references to every tensor whose address is in the argument struct, and the workspace the plan fills."""
+    Holds references to every tensor whose address is in the argument struct, and the workspace the plan fills. The
+    backward follows the forward (the BF16 relu backward reads what the forward left in the workspace), as in the
+    reference's autograd; calling it first raises."""
 
     def __init__(self, pos_min, pos_max, prefix, origin, direction, segs, grid, params, mlp, jitter=False, seed=0,
                  stream=None, precision="fp32", grid_bf16=None):
@@ -232,6 +234,7 @@ class Projector:
         self.grid_bf16 = grid_bf16
         self.args = a
         self.grid, self.params, self.R, self.S = grid, params, R, S
+        self._forwarded = False
         nbytes = _abi.lib().neat_projector_workspace_bytes(ctypes.byref(a))
         if nbytes == 0:
             raise _abi.NeatError(_abi.NEAT_E_UNSUPPORTED, "neat_projector_workspace_bytes", _abi.last_error())
@@ -242,10 +245,16 @@ class Projector:
         proj = torch.empty(self.R, dtype=_F32, device=self.grid.device)
         call("neat_projector_fwd", stream_ptr(self.stream), ctypes.byref(self.args), int(bool(compensated)), ptr(proj),
              ptr(self.ws), self.ws.numel())
+        self._forwarded = True
         return proj
+
+    def _require_forward(self, what):
+        if not self._forwarded:
+            raise RuntimeError(f"Projector.{what}: run forward() first (the backward reads what it leaves)")
 
     def backward_stage(self, stage, grad_proj, grad_grid, grad_params):
         """One stage of the backward (1 decoder, 2 parameter gradient, 3 grid gradient); 1..3 in order = backward."""
+        self._require_forward("backward_stage")
         call("neat_projector_bwd_stage", stream_ptr(self.stream), ctypes.byref(self.args), int(stage),
              ptr(grad_proj) if grad_proj is not None else None, ptr(grad_grid), ptr(grad_params), ptr(self.ws),
              self.ws.numel())
@@ -256,6 +265,7 @@ class Projector:
         off = torch.empty(max(self.args.num_segments, 1), dtype=torch.int64, device=self.grid.device)
         call("neat_projector_sample_values", stream_ptr(self.stream), ctypes.byref(self.args), ptr(values), ptr(off),
              ptr(self.ws), self.ws.numel())
+        self._forwarded = True  # the same forward kernels, leaving the same per-sample cache
         return values, off
 
     def ray_grad(self):
@@ -268,6 +278,7 @@ class Projector:
         return go, gd
 
     def backward(self, grad_proj):
+        self._require_forward("backward")
         grad_proj = grad_proj.contiguous()
         gg = torch.empty_like(self.grid)
         gp = torch.empty_like(self.params)

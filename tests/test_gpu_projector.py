@@ -123,6 +123,24 @@ def test_fused_projector_deterministic_forward_and_empty():
     assert torch.all(proj == 0) and torch.all(gg == 0) and torch.all(gp == 0)
 
 
+def test_backward_before_forward_raises():
+    """The BF16 relu backward reads the features, weights and cells its forward left in the workspace, so the
+    Python mirror refuses a backward that no forward preceded (before any kernel of the backward runs)."""
+    print("case: backward before forward", flush=True)
+    cfg = W.C1
+    sc, ds, o, d, to, td = setup(cfg, 256)
+    segs = pipeline.sample(ds, to, td, jitter=False, emit=False)
+    pj = ops.Projector(ds.pos_min, ds.pos_max, ds.prefix, to, td, segs, ds.grid, ds.params, ds.mlp, False, 0,
+                       precision="bf16")
+    with pytest.raises(RuntimeError, match="forward"):
+        pj.backward(torch.ones(to.shape[0], device=DEV))
+    with pytest.raises(RuntimeError, match="forward"):
+        pj.backward_stage(1, torch.ones(to.shape[0], device=DEV), None, None)
+    pj.forward()
+    gg, gp = pj.backward(torch.ones(to.shape[0], device=DEV))
+    assert torch.isfinite(gg).all() and torch.isfinite(gp).all()
+
+
 @pytest.mark.parametrize("cfg_name,precision", [("C1", "fp32"), ("C2", "fp32"), ("C2", "bf16")])
 def test_ray_gradient_vs_oracle(oracle, cfg_name, precision):
     """The fused projector's positional gradient (neat_projector_ray_grad: dL/dorigin, dL/ddirection with the samples'
