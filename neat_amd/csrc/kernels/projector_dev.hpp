@@ -16,9 +16,10 @@ constexpr int kSegCap     = kFwdChunk / kSegCost;  // segments per forward chunk
 constexpr int kFwdThreads = 256;
 
 // Node-resident workgroups (the BF16 relu kernels): a workgroup of kNodeThreads (one fits a CU) walks a contiguous,
-// cost-balanced run of forward chunks and keeps the current slot's BF16 node grid in LDS, so every corner gather
-// reads LDS and each node grid is fetched from HBM about once per pass. There are kNodeWGPerCU runs per CU, so the
-// hardware's dispatch of the next waiting workgroup to the first free CU evens out what the cost model misses.
+// cost-balanced run of forward chunks. The forward keeps the current slot's BF16 node grid in LDS, so every corner
+// gather reads LDS and each node grid is fetched from HBM about once per pass; the backward reads the per-sample
+// features the forward left (Plan::feat_cache). There are kNodeWGPerCU runs per CU, so the hardware's dispatch of
+// the next waiting workgroup to the first free CU evens out what the cost model misses.
 constexpr int kNodeThreads  = 512;
 constexpr int kNodeWaves    = kNodeThreads / 64;
 constexpr int kNodeWGPerCU  = 4;
